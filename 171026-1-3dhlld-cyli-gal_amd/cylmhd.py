@@ -1,0 +1,279 @@
+"""ctypes binding of libcylmhd.so (the C ABI declared in include/cylmhd.h).
+
+The library is the product: hand-written gfx950 HIP kernels behind a C ABI.
+This module only marshals numpy arrays across that ABI; it never computes
+the step itself and it refuses to run when the library is missing, so a test
+or benchmark can never pass on a silent fallback.
+
+Arrays use the reference layout A[m][i][j][k] (m = 9 components, k fastest,
+ghosts included), FP64, C-contiguous.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcylmhd.so")
+
+LIMITER_MINMOD, LIMITER_MC = 0, 1
+SOLVER_HLLD, SOLVER_HLL = 0, 1
+COOL_NONE, COOL_RAYMOND, COOL_POWERLAW = 0, 1, 2
+GRAV_GALACTIC, GRAV_POINTMASS = 0, 1
+PATH_PLAIN64, PATH_FUSED64, PATH_FUSED32 = 0, 1, 2
+PATH_NAMES = {PATH_PLAIN64: "plain64", PATH_FUSED64: "fused64", PATH_FUSED32: "fused32"}
+
+
+class Params(C.Structure):
+    """cylmhd_params (include/cylmhd.h); field order must match the header."""
+    _fields_ = [
+        ("ix", C.c_int), ("jy", C.c_int), ("kz", C.c_int),
+        ("lx", C.c_double), ("lz", C.c_double),
+        ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+        ("dx0", C.c_double), ("dy0", C.c_double), ("dz0", C.c_double),
+        ("stretchratio", C.c_double), ("dxmax", C.c_double), ("dzmax", C.c_double),
+        ("rin", C.c_double),
+        ("cfl", C.c_double), ("dtmin", C.c_double),
+        ("rhocr", C.c_double), ("prcr", C.c_double), ("betacr", C.c_double),
+        ("gm", C.c_double), ("gm1", C.c_double), ("oneogm1", C.c_double),
+        ("cr", C.c_double), ("te0", C.c_double),
+        ("epssp0", C.c_double), ("tausp", C.c_double), ("scalez", C.c_double),
+        ("scaler", C.c_double), ("scalephase", C.c_double),
+        ("armnum", C.c_int),
+        ("omegasp", C.c_double), ("omegapattern", C.c_double), ("pitch", C.c_double),
+        ("limiter", C.c_int), ("solver", C.c_int), ("rk", C.c_int),
+        ("cooling", C.c_int), ("gravity", C.c_int),
+        ("gm_point", C.c_double),
+        ("cool_lambda0", C.c_double), ("cool_tmin", C.c_double), ("cool_index", C.c_double),
+        ("path", C.c_int),
+        ("disc_h", C.c_double), ("disc_floor", C.c_double), ("disc_cs2", C.c_double),
+        ("disc_beta", C.c_double), ("beta_lo", C.c_double), ("beta_hi", C.c_double),
+        ("beta_mode", C.c_int),
+        ("pert_amp", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+    def copy(self):
+        p = Params()
+        C.memmove(C.byref(p), C.byref(self), C.sizeof(Params))
+        return p
+
+    @property
+    def dims(self):
+        return self.ix + 2, self.jy + 2, self.kz + 2
+
+
+class Diag(C.Structure):
+    _fields_ = [("mass", C.c_double), ("energy", C.c_double),
+                ("divb_l1", C.c_double), ("psi_l1", C.c_double)]
+
+
+class CylmhdError(RuntimeError):
+    pass
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load libcylmhd.so once; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.isfile(LIB_PATH):
+        raise CylmhdError(f"{LIB_PATH} is missing: run __graft_entry__.build() (there is no fallback)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER(Params)
+    sig = {
+        "cylmhd_params_default": ([P], C.c_int),
+        "cylmhd_params_config": ([P, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "cylmhd_grid_host": ([P, _dp, _dp, _dp], C.c_int),
+        "cylmhd_model_disc": ([P, _dp], C.c_int),
+        "cylmhd_model_galactic_disc3": ([P, _dp], C.c_int),
+        "cylmhd_vtou_host": ([P, _dp, _dp], C.c_int),
+        "cylmhd_create": ([P, C.POINTER(C.c_void_p)], C.c_int),
+        "cylmhd_destroy": ([C.c_void_p], C.c_int),
+        "cylmhd_dims": ([C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "cylmhd_set_initial": ([C.c_void_p, _dp, _dp], C.c_int),
+        "cylmhd_upload_state": ([C.c_void_p, _dp, _dp], C.c_int),
+        "cylmhd_download_state": ([C.c_void_p, _dp, _dp], C.c_int),
+        "cylmhd_setdt": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                          C.POINTER(C.c_int)], C.c_int),
+        "cylmhd_gravity": ([C.c_void_p, C.c_double], C.c_int),
+        "cylmhd_step": ([C.c_void_p, C.c_double, C.c_double], C.c_int),
+        "cylmhd_floor_counts": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
+        "cylmhd_diagnostics": ([C.c_void_p, C.POINTER(Diag)], C.c_int),
+        "cylmhd_sync": ([C.c_void_p], C.c_int),
+        "cylmhd_kernel_timing": ([C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_int)], C.c_int),
+        "cylmhd_error_string": ([C.c_int], C.c_char_p),
+        "cylmhd_binary_output": ([C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                  _dp, _dp, _dp, _dp], C.c_int),
+        "cylmhd_restart_read": ([C.c_char_p, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        try:
+            fn = getattr(L, name)
+        except AttributeError:
+            continue          # reported by missing_symbols(); calling it raises
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def missing_symbols():
+    L = lib()
+    return [s for s in exported_symbols() if not hasattr(L, s)]
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().cylmhd_error_string(rc).decode()
+        raise CylmhdError(f"{what}: {msg} ({rc})")
+
+
+def params_default():
+    p = Params()
+    check(lib().cylmhd_params_default(C.byref(p)), "params_default")
+    return p
+
+
+def params_config(config, nr=0, nphi=0, nz=0):
+    p = Params()
+    check(lib().cylmhd_params_config(C.byref(p), config, nr, nphi, nz), "params_config")
+    return p
+
+
+def grid(p):
+    NX, NY, NZ = p.dims
+    x, y, z = np.zeros(NX), np.zeros(NY), np.zeros(NZ)
+    check(lib().cylmhd_grid_host(C.byref(p), x, y, z), "grid")
+    return x, y, z
+
+
+def model_disc(p):
+    NX, NY, NZ = p.dims
+    V = np.zeros((9, NX, NY, NZ))
+    check(lib().cylmhd_model_disc(C.byref(p), V), "model_disc")
+    return V
+
+
+def model_galactic_disc3(p):
+    NX, NY, NZ = p.dims
+    V = np.zeros((9, NX, NY, NZ))
+    check(lib().cylmhd_model_galactic_disc3(C.byref(p), V), "model_galactic_disc3")
+    return V
+
+
+def vtou(p, V):
+    U = np.zeros_like(V)
+    check(lib().cylmhd_vtou_host(C.byref(p), np.ascontiguousarray(V), U), "vtou")
+    return U
+
+
+def binary_output(dirname, t, x, y, z, V):
+    nx, ny, nz = V.shape[1:]
+    check(lib().cylmhd_binary_output(dirname.encode(), nx, ny, nz, t, x, y, z,
+                                     np.ascontiguousarray(V)), "binary_output")
+
+
+def restart_read(dirname, shape):
+    V = np.zeros((9,) + tuple(shape))
+    check(lib().cylmhd_restart_read(dirname.encode(), shape[0], shape[1], shape[2], V), "restart_read")
+    return V
+
+
+class Context:
+    """One device context (one HIP stream).  Mirrors the reference call
+    sequence: set_initial (SaveInitial), setdt (Setdt), gravity
+    (SpiralGravity), step (TVDRK)."""
+
+    def __init__(self, p):
+        self.p = p.copy()
+        self.dims = p.dims
+        h = C.c_void_p()
+        check(lib().cylmhd_create(C.byref(self.p), C.byref(h)), "create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().cylmhd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_initial(self, U, V):
+        check(lib().cylmhd_set_initial(self.h, np.ascontiguousarray(U), np.ascontiguousarray(V)), "set_initial")
+
+    def upload(self, U, V):
+        check(lib().cylmhd_upload_state(self.h, np.ascontiguousarray(U), np.ascontiguousarray(V)), "upload")
+
+    def download(self):
+        U = np.zeros((9,) + self.dims)
+        V = np.zeros((9,) + self.dims)
+        check(lib().cylmhd_download_state(self.h, U, V), "download")
+        return U, V
+
+    def setdt(self):
+        dt, ch = C.c_double(), C.c_double()
+        ijk = (C.c_int * 3)()
+        check(lib().cylmhd_setdt(self.h, C.byref(dt), C.byref(ch), ijk), "setdt")
+        return dt.value, ch.value, tuple(ijk)
+
+    def gravity(self, t):
+        check(lib().cylmhd_gravity(self.h, t), "gravity")
+
+    def step(self, dt, ch):
+        check(lib().cylmhd_step(self.h, dt, ch), "step")
+
+    def sync(self):
+        check(lib().cylmhd_sync(self.h), "sync")
+
+    def floor_counts(self):
+        c = (C.c_int * 9)()
+        check(lib().cylmhd_floor_counts(self.h, c), "floor_counts")
+        return np.array(list(c)).reshape(3, 3)
+
+    def diagnostics(self):
+        d = Diag()
+        check(lib().cylmhd_diagnostics(self.h, C.byref(d)), "diagnostics")
+        return dict(mass=d.mass, energy=d.energy, divb_l1=d.divb_l1, psi_l1=d.psi_l1)
+
+    def kernel_timing(self, which=0, reset=False):
+        ms, n = C.c_double(), C.c_int()
+        check(lib().cylmhd_kernel_timing(self.h, which, int(reset), C.byref(ms), C.byref(n)), "kernel_timing")
+        return ms.value, n.value
+
+    def run(self, nsteps, t=0.0):
+        """main.c:316-393 loop (no dumps): returns (t, [dt...])."""
+        dts = []
+        for _ in range(nsteps):
+            dt, ch, _ = self.setdt()
+            if not np.isfinite(dt) or dt < self.p.dtmin:
+                raise CylmhdError(f"stop due to small dt or inf (dt={dt})")
+            self.gravity(t)
+            self.step(dt, ch)
+            t += dt
+            dts.append(dt)
+        return t, dts
+
+
+def exported_symbols():
+    """Names declared in include/cylmhd.h (for the export test)."""
+    import re
+    hdr = os.path.join(HERE, "..", "include", "cylmhd.h")
+    with open(hdr) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"\b(cylmhd_[a-z0-9_]+)\s*\(", txt)))

@@ -1,0 +1,169 @@
+/*
+ * cylmhd.h -- C ABI of the MI355X-native cylindrical HLLD MHD time step.
+ *
+ * The reference (github-nakasho/171026-1-3dhlld-cyli-gal) has no API: every
+ * routine is `int f(void)` over file-scope globals (commonfield.h:114-141,
+ * commontemp.h:160-184, commongrid.h:42-62) sized by compile-time #defines
+ * (input.h:45-72).  The entry points below replace exactly the calls that
+ * main.c makes on the hot path and at start-up; each declaration names the
+ * reference routine it stands in for.  All arrays crossing this boundary are
+ * host FP64 arrays in the reference layout  A[m][i][j][k]  (component-major,
+ * k fastest, ghosts included: NX = ix+2, NY = jy+2, NZ = kz+2), so a driver
+ * written against the reference globals can hand its arrays over unchanged
+ * and the binary dumps stay byte-identical.
+ *
+ * Every function returns 0 on success or a negative CYLMHD_E* code; nothing
+ * in the library calls exit().  One context = one GPU stream; a context is
+ * not re-entrant (like the reference's globals) but contexts are independent.
+ */
+#ifndef CYLMHD_H
+#define CYLMHD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define CYLMHD_OK          0
+#define CYLMHD_EINVAL     -1   /* bad argument / inconsistent parameters   */
+#define CYLMHD_ENOMEM     -2   /* device or host allocation failed         */
+#define CYLMHD_EHIP       -3   /* a HIP runtime call failed                */
+#define CYLMHD_EIO        -4   /* dump / restart file could not be opened  */
+#define CYLMHD_EUNSUP     -5   /* option not available on this path        */
+
+/* ---- scheme selectors (the reference picks these by commenting calls) -- */
+/* limiter: firststep.h:37-38 (MinmodLimiterX is the active one)          */
+#define CYLMHD_LIMITER_MINMOD 0   /* limiterx.h:171-217 (MUSCL-Hancock form) */
+#define CYLMHD_LIMITER_MC     1   /* limiterx.h:335-385                      */
+/* Riemann solver: firststep.h:45-46                                       */
+#define CYLMHD_SOLVER_HLLD    0   /* hlldx.h:28-499                          */
+#define CYLMHD_SOLVER_HLL     1   /* hllx.h:29-184                           */
+/* cooling law used by the energy source and the CFL limit                 */
+#define CYLMHD_COOL_NONE      0
+#define CYLMHD_COOL_RAYMOND   1   /* cooling.h:22-78 (int-truncated)         */
+#define CYLMHD_COOL_POWERLAW  2   /* Lambda = lambda0 * T^index for T > tmin */
+/* axisymmetric gravity model                                              */
+#define CYLMHD_GRAV_GALACTIC  0   /* initialgravity.h:22-79                  */
+#define CYLMHD_GRAV_POINTMASS 1   /* -GM r_vec/|r|^3 (Keplerian discs)       */
+/* device path                                                             */
+#define CYLMHD_PATH_PLAIN64   0   /* pass-by-pass FP64, reference data flow  */
+#define CYLMHD_PATH_FUSED64   1   /* one fused kernel per RK stage, FP64     */
+#define CYLMHD_PATH_FUSED32   2   /* fused, FP32 state + FP64 recovery/sums  */
+
+/* ---- run-time parameter set: input.h:45-102 + constant.h:62-89 --------- */
+typedef struct cylmhd_params {
+    /* grid (input.h:45-76); NX = ix+2, NY = jy+2, NZ = kz+2 (margin = 2) */
+    int    ix, jy, kz;
+    double lx, lz;               /* start of the stretched zone in r and z */
+    double x0, y0, z0;           /* first interior cell centre             */
+    double dx0, dy0, dz0;        /* base spacings                          */
+    double stretchratio, dxmax, dzmax;
+    double rin;                  /* central absorbing radius               */
+    /* run control (input.h:58-59, 78-81) */
+    double cfl, dtmin;
+    double rhocr, prcr, betacr;
+    /* constants (constant.h:62-79, 189-196) */
+    double gm, gm1, oneogm1;     /* 5/3, 2/3, 1.5 as separate literals     */
+    double cr;                   /* GLM: psi damping exp(-dt ch/cr/2)      */
+    double te0;                  /* temperature unit of Cooling()          */
+    /* spiral arm / rotating frame (input.h:88-96) */
+    double epssp0, tausp, scalez, scaler, scalephase;
+    int    armnum;
+    double omegasp, omegapattern, pitch;
+    /* scheme selection */
+    int    limiter, solver, rk;  /* rk = 3 (FirstStep..ThirdStep) or 2     */
+    int    cooling, gravity;
+    double gm_point;             /* GM of CYLMHD_GRAV_POINTMASS            */
+    double cool_lambda0, cool_tmin, cool_index;
+    int    path;                 /* CYLMHD_PATH_*                          */
+    /* synthetic disc initial state (benchmark discs; host-side only) */
+    double disc_h, disc_floor, disc_cs2, disc_beta, beta_lo, beta_hi;
+    int    beta_mode;            /* 0: constant disc_beta, 1: log-uniform per annulus */
+    double pert_amp;
+    uint64_t seed;
+} cylmhd_params;
+
+/* Fill *p with the reference's own input.h / constant.h values. */
+int cylmhd_params_default(cylmhd_params *p);
+
+/* Fill *p for one of the benchmark discs:
+ *   1: 64x64x32, minmod, CPU-sized;     2: 256x256x128, MC, spiral 5%;
+ *   3: 512x512x256, MC, beta log-uniform [0.1,100] per annulus;
+ *   5: 768x768x384, MC, power-law cooling, spiral 10%, GLM alpha 0.1.
+ * nr/nphi/nz > 0 override the grid size (same disc, other resolution). */
+int cylmhd_params_config(cylmhd_params *p, int config, int nr, int nphi, int nz);
+
+/* ---- host-side set-up (no GPU) ----------------------------------------- */
+/* Grid: cylindricalgrid.h:25-227.  x[NX], y[NY], z[NZ] cell centres. */
+int cylmhd_grid_host(const cylmhd_params *p, double *x, double *y, double *z);
+/* Synthetic disc primitives V[9][NX][NY][NZ] (all cells incl. ghosts). */
+int cylmhd_model_disc(const cylmhd_params *p, double *V);
+/* GalacticDisc3 (model.h:149-327) primitives, without the U-side z BCs. */
+int cylmhd_model_galactic_disc3(const cylmhd_params *p, double *V);
+/* VtoU (vtou.h:20-62) on host arrays. */
+int cylmhd_vtou_host(const cylmhd_params *p, const double *V, double *U);
+
+/* ---- device context: the hot path --------------------------------------- */
+typedef struct cylmhd_ctx cylmhd_ctx;
+
+/* Allocates device state; builds grid, geometry and gravity tables
+ * (CylindricalGrid cylindricalgrid.h:25, InitialGravity initialgravity.h:22). */
+int cylmhd_create(const cylmhd_params *p, cylmhd_ctx **out);
+int cylmhd_destroy(cylmhd_ctx *c);
+int cylmhd_dims(const cylmhd_ctx *c, int *nx, int *ny, int *nz);
+
+/* SaveInitial (saveinitial.h:18-50): U,V become the current state AND the
+ * Uinitial/Vinitial/rhofloor references of the absorbing layers and floors. */
+int cylmhd_set_initial(cylmhd_ctx *c, const double *U, const double *V);
+/* Replace the current state (restart: RestartRead restartread.h:9-143 after
+ * its VtoU + Check, or any state the caller holds). */
+int cylmhd_upload_state(cylmhd_ctx *c, const double *U, const double *V);
+/* Copy the current state back (V feeds BinaryOutput, output.h:1202-1414). */
+int cylmhd_download_state(cylmhd_ctx *c, double *U, double *V);
+
+/* Setdt (setdt.h:24-94): dt = cfl/max, ch = minminlength*max, and the
+ * (i,j,k) of the limiting cell ("dt factor mesh").  main.c:324. */
+int cylmhd_setdt(cylmhd_ctx *c, double *dt, double *ch, int ijk[3]);
+/* SpiralGravity (spiralgravity.h:20-124) at time t.  main.c:341. */
+int cylmhd_gravity(cylmhd_ctx *c, double t);
+/* TVDRK (tvdrungekutta.h:16-78): one full time step of dt with GLM speed
+ * ch; RK3 = FirstStep/SecondStep/ThirdStep, RK2 = Halfstep/Fullstep.
+ * Enqueued on the context stream; returns without waiting.  main.c:348. */
+int cylmhd_step(cylmhd_ctx *c, double dt, double ch);
+/* Check() floor counters (check.h:180-183) of the stages of the last step:
+ * counts[3*s + 0..2] = rho, pressure, beta hits of stage s. */
+int cylmhd_floor_counts(cylmhd_ctx *c, int counts[9]);
+
+typedef struct cylmhd_diag {
+    double mass;      /* sum U0 dV over interior cells                    */
+    double energy;    /* sum U7 dV                                        */
+    double divb_l1;   /* sum |div B| dV (face-averaged finite volume)     */
+    double psi_l1;    /* sum |psi| dV                                     */
+} cylmhd_diag;
+/* Conserved totals and div B (not in the reference; output.h:577-638 is a
+ * commented-out Cartesian divB).  FP64 on every path. */
+int cylmhd_diagnostics(cylmhd_ctx *c, cylmhd_diag *d);
+
+/* Wait for the context stream; returns the first asynchronous error. */
+int cylmhd_sync(cylmhd_ctx *c);
+/* Average duration (ms) of the named kernel class over the launches since
+ * the last reset, timed with HIP events on the context stream:
+ * which = 0: the per-stage update kernel(s) of the active path. */
+int cylmhd_kernel_timing(cylmhd_ctx *c, int which, int reset, double *avg_ms, int *launches);
+const char *cylmhd_error_string(int code);
+
+/* ---- dump / restart byte format (host only) ---------------------------- */
+/* BinaryOutput (output.h:1202-1414): ./<dir>/GridTime.dat + r, phi, z,
+ * rho, vr, vphi, vz, br, bphi, bz, pr, psi as raw native-endian FP64. */
+int cylmhd_binary_output(const char *dir, int nx, int ny, int nz, double t,
+                         const double *x, const double *y, const double *z,
+                         const double *V);
+/* RestartRead (restartread.h:9-143): reads the nine primitive files. */
+int cylmhd_restart_read(const char *dir, int nx, int ny, int nz, double *V);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CYLMHD_H */
