@@ -1,0 +1,446 @@
+/*
+ * api.hip -- the C ABI of include/cylmhd.h on top of the device paths.
+ *
+ * A context owns one HIP stream (default priority), the geometry / gravity
+ * tables and the state of its path.  Host arrays cross the boundary in the
+ * reference layout [9][NX][NY][NZ] FP64.  No function exits the process;
+ * HIP failures come back as CYLMHD_EHIP with the HIP message kept for
+ * cylmhd_error_string.
+ */
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "fused.h"
+#include "paths.h"
+
+using namespace cyl;
+
+namespace {
+thread_local char g_errmsg[512] = "";
+
+int hip_fail(hipError_t e, const char *what)
+{
+    snprintf(g_errmsg, sizeof g_errmsg, "%s: %s", what, hipGetErrorString(e));
+    return CYLMHD_EHIP;
+}
+#define HIPCHK(call)                                        \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);   \
+    } while (0)
+
+const int kEventPairs = 96;
+}  // namespace
+
+struct cylmhd_ctx {
+    cylmhd_params p;
+    cyl_grid g;
+    cyl_spiral_tables sp;
+    hipStream_t stream = nullptr;
+    DevGeom dg{};
+    DevState ds{};
+    FusedState fs{};
+    std::vector<void *> allocs;
+    double *d_sincos = nullptr, *d_logxs = nullptr;
+    bool have_state = false;
+    /* timing of the per-stage hot kernels */
+    bool timing = false;
+    hipEvent_t ev[kEventPairs][2];
+    int ev_used = 0;
+    bool ev_ready = false;
+};
+
+namespace {
+
+template <typename T>
+int dalloc(cylmhd_ctx *c, T **ptr, size_t count)
+{
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T) + 256);
+    if (e != hipSuccess) {
+        snprintf(g_errmsg, sizeof g_errmsg, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+        return CYLMHD_ENOMEM;
+    }
+    c->allocs.push_back(p);
+    *ptr = (T *)p;
+    return CYLMHD_OK;
+}
+
+template <typename T>
+int dupload(cylmhd_ctx *c, const T **dst, const T *src, size_t count)
+{
+    T *p;
+    int rc = dalloc(c, &p, count);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    *dst = p;
+    return CYLMHD_OK;
+}
+
+Consts make_consts(const cylmhd_ctx *c, double dt, double ch)
+{
+    Consts k{};
+    k.gm = c->p.gm; k.gm1 = c->p.gm1; k.oneogm1 = c->p.oneogm1;
+    k.dt = dt; k.ch = ch;
+    k.psicoef = exp(-dt * ch * (1.0 / c->p.cr) * 0.5);   /* tvdrungekutta.h:22 */
+    k.prcr = c->p.prcr; k.betacr = c->p.betacr; k.oneobetacr = (1.0 / c->p.betacr); k.rhocr = c->p.rhocr;
+    k.omegasp = c->p.omegasp;
+    k.te0 = c->p.te0;
+    k.cooling = c->p.cooling;
+    k.cool_lambda0 = c->p.cool_lambda0; k.cool_tmin = c->p.cool_tmin; k.cool_index = c->p.cool_index;
+    return k;
+}
+
+int validate(const cylmhd_params *p)
+{
+    if (!p) return CYLMHD_EINVAL;
+    if (p->ix < 12 || p->jy < 2 || (p->jy & 1) || p->kz < 2) {
+        snprintf(g_errmsg, sizeof g_errmsg, "grid %dx%dx%d: need ix >= 12, even jy >= 2, kz >= 2", p->ix, p->jy, p->kz);
+        return CYLMHD_EINVAL;
+    }
+    if (p->rk != 2 && p->rk != 3) { snprintf(g_errmsg, sizeof g_errmsg, "rk must be 2 or 3"); return CYLMHD_EINVAL; }
+    if (p->rk == 2 && p->kz < 9) {
+        /* ZRightAbsorbBoundary reaches k = kzmax1-10 (zboundary.h:33-58) */
+        snprintf(g_errmsg, sizeof g_errmsg, "RK2 needs kz >= 9 (z absorbing layer of 11 cells)");
+        return CYLMHD_EINVAL;
+    }
+    if ((int)(p->rin / p->dx0) + 1 > p->ix + 2) { snprintf(g_errmsg, sizeof g_errmsg, "rin too large"); return CYLMHD_EINVAL; }
+    if (p->path < CYLMHD_PATH_PLAIN64 || p->path > CYLMHD_PATH_FUSED32) return CYLMHD_EINVAL;
+    return CYLMHD_OK;
+}
+
+void destroy_ctx(cylmhd_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->ev_ready)
+        for (int q = 0; q < kEventPairs; q++) { hipEventDestroy(c->ev[q][0]); hipEventDestroy(c->ev[q][1]); }
+    for (void *p : c->allocs) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    cyl_grid_free(&c->g);
+    cyl_spiral_tables_free(&c->sp);
+    delete c;
+}
+
+int build_tables(cylmhd_ctx *c)
+{
+    const cylmhd_params *p = &c->p;
+    cyl_grid &g = c->g;
+    int rc;
+    if ((rc = cyl_grid_build(p, &g))) return rc;
+    if ((rc = cyl_spiral_tables_build(p, &g, &c->sp))) return rc;
+    const int NX = g.NX, NY = g.NY, NZ = g.NZ;
+    std::vector<double> grx((size_t)NX * NZ), grz((size_t)NX * NZ);
+    if ((rc = cyl_gravity_axi(p, &g, grx.data(), grz.data()))) return rc;
+    DevGeom &d = c->dg;
+    d.NX = NX; d.NY = NY; d.NZ = NZ; d.jy = p->jy; d.irin = g.irin;
+    d.oneody = g.oneody;
+    const size_t T = (size_t)NX * NZ;
+    if ((rc = dupload(c, &d.x, g.x, NX)) || (rc = dupload(c, &d.xm, g.xm, NX)) ||
+        (rc = dupload(c, &d.oneox, g.oneox, NX)) || (rc = dupload(c, &d.oneodx, g.oneodx, NX)) ||
+        (rc = dupload(c, &d.oneodz, g.oneodz, NZ)) || (rc = dupload(c, &d.y, g.y, NY)) ||
+        (rc = dupload(c, &d.z, g.z, NZ)) || (rc = dupload(c, &d.g1, g.g1, T)) ||
+        (rc = dupload(c, &d.g2, g.g2, T)) || (rc = dupload(c, &d.g3, g.g3, T)) ||
+        (rc = dupload(c, &d.g5, g.g5, T)) || (rc = dupload(c, &d.dvol, g.dvol, T)) ||
+        (rc = dupload(c, &d.minlength, g.minlength, T)) ||
+        (rc = dupload(c, &d.grxaxi, (const double *)grx.data(), T)) ||
+        (rc = dupload(c, &d.grzaxi, (const double *)grz.data(), T)) ||
+        (rc = dupload(c, &d.sp_x2s2, c->sp.x2s2, NX)) || (rc = dupload(c, &d.sp_o1x2s2, c->sp.o1x2s2, NX)) ||
+        (rc = dupload(c, &d.sp_xs2o15, c->sp.xs2o15, NX)) || (rc = dupload(c, &d.sp_x2s2o15, c->sp.x2s2o15, NX)) ||
+        (rc = dupload(c, &d.sp_szs, c->sp.szs, NZ)) || (rc = dupload(c, &d.sp_onepa, c->sp.onepa, NZ)) ||
+        (rc = dupload(c, &d.sp_oz2s2, c->sp.oz2s2, NZ)) || (rc = dupload(c, &d.sp_q, c->sp.q, T)) ||
+        (rc = dupload(c, &d.xabs, (const double *)g.xabs, 11)) || (rc = dupload(c, &d.zabs, (const double *)g.zabs, 11)) ||
+        (rc = dupload(c, &d.cabs, g.cabs, (size_t)(g.irin + 1) * NZ)))
+        return rc;
+    const double *lx;
+    if ((rc = dupload(c, &lx, c->sp.logxs, NX))) return rc;
+    c->d_logxs = (double *)lx;
+    if ((rc = dalloc(c, &c->d_sincos, 2 * (size_t)NX * NY))) return rc;
+    HIPCHK(hipMemset(c->d_sincos, 0, 2 * sizeof(double) * NX * NY));
+    d.sincos = c->d_sincos;
+    d.cotpitch = c->sp.cotpitch;
+    d.armnum = p->armnum;
+    d.epssp = 0.0;
+    /* absorbing i-planes: [0, irin] and [NX-11, NX-1] */
+    std::vector<int> il, ol;
+    for (int i = 0; i < NX; i++) {
+        if (i <= g.irin || i >= NX - 11) il.push_back(i);
+        else ol.push_back(i);
+    }
+    const int *pil, *pol;
+    if ((rc = dupload(c, &pil, (const int *)il.data(), il.size()))) return rc;
+    if ((rc = dupload(c, &pol, (const int *)(ol.empty() ? il.data() : ol.data()), ol.empty() ? 1 : ol.size()))) return rc;
+    c->ds.absorb_i = (int *)pil; c->ds.n_absorb_i = (int)il.size();
+    c->ds.nonabsorb_i = (int *)pol; c->ds.n_nonabsorb_i = (int)ol.size();
+    return CYLMHD_OK;
+}
+
+int alloc_state(cylmhd_ctx *c)
+{
+    DevState &d = c->ds;
+    const int NX = c->g.NX, NY = c->g.NY, NZ = c->g.NZ;
+    d.N = (size_t)NX * NY * NZ;
+    d.NF = (size_t)(NX - 1) * (NY - 1) * (NZ - 1);
+    int rc;
+    if ((rc = dalloc(c, &d.Uinit, 9 * d.N)) || (rc = dalloc(c, &d.Vinit_z, 9 * 2 * (size_t)NX * NY)) ||
+        (rc = dalloc(c, &d.counts, 9)))
+        return rc;
+    d.nred = 8192;
+    if ((rc = dalloc(c, &d.red, (size_t)d.nred))) return rc;
+    HIPCHK(hipMemset(d.counts, 0, 9 * sizeof(int)));
+    if (c->p.path == CYLMHD_PATH_PLAIN64) {
+        if ((rc = dalloc(c, &d.U, 9 * d.N)) || (rc = dalloc(c, &d.U1, 9 * d.N)) || (rc = dalloc(c, &d.V, 9 * d.N)) ||
+            (rc = dalloc(c, &d.F, 9 * d.NF)) || (rc = dalloc(c, &d.G, 9 * d.NF)) || (rc = dalloc(c, &d.H, 9 * d.NF)))
+            return rc;
+    } else {
+        /* fused paths keep no V and no face arrays; V is only materialised
+         * on demand (Setdt, download) into a scratch array */
+        if ((rc = dalloc(c, &d.V, 9 * d.N))) return rc;
+        if ((rc = fused_alloc(c->fs, c->dg, d, c->p.path == CYLMHD_PATH_FUSED32,
+                              [c](size_t bytes, void **ptr) {
+                                  char *q;
+                                  int r = dalloc(c, &q, bytes);
+                                  *ptr = q;
+                                  return r;
+                              })))
+            return rc;
+    }
+    return CYLMHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *cylmhd_error_string(int code)
+{
+    static thread_local char buf[640];
+    const char *base = code == CYLMHD_OK ? "ok" : code == CYLMHD_EINVAL ? "invalid argument"
+                     : code == CYLMHD_ENOMEM ? "out of memory" : code == CYLMHD_EHIP ? "HIP error"
+                     : code == CYLMHD_EIO ? "I/O error" : code == CYLMHD_EUNSUP ? "unsupported" : "unknown error";
+    snprintf(buf, sizeof buf, "%s%s%s", base, g_errmsg[0] ? ": " : "", g_errmsg);
+    return buf;
+}
+
+int cylmhd_create(const cylmhd_params *p, cylmhd_ctx **out)
+{
+    if (!out) return CYLMHD_EINVAL;
+    *out = nullptr;
+    g_errmsg[0] = 0;
+    int rc = validate(p);
+    if (rc) return rc;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev < 1) {
+        snprintf(g_errmsg, sizeof g_errmsg, "no HIP device (%s)", hipGetErrorString(e));
+        return CYLMHD_EHIP;
+    }
+    cylmhd_ctx *c = new cylmhd_ctx();
+    c->p = *p;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+        rc = hip_fail(e, "hipStreamCreate");
+        destroy_ctx(c);
+        return rc;
+    }
+    if ((rc = build_tables(c)) || (rc = alloc_state(c))) {
+        destroy_ctx(c);
+        return rc;
+    }
+    *out = c;
+    return CYLMHD_OK;
+}
+
+int cylmhd_destroy(cylmhd_ctx *c)
+{
+    destroy_ctx(c);
+    return CYLMHD_OK;
+}
+
+int cylmhd_dims(const cylmhd_ctx *c, int *nx, int *ny, int *nz)
+{
+    if (!c) return CYLMHD_EINVAL;
+    if (nx) *nx = c->g.NX;
+    if (ny) *ny = c->g.NY;
+    if (nz) *nz = c->g.NZ;
+    return CYLMHD_OK;
+}
+
+int cylmhd_upload_state(cylmhd_ctx *c, const double *U, const double *V)
+{
+    if (!c || !U || !V) return CYLMHD_EINVAL;
+    const size_t bytes = 9 * c->ds.N * sizeof(double);
+    if (c->p.path == CYLMHD_PATH_PLAIN64) {
+        HIPCHK(hipMemcpyAsync(c->ds.U, U, bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->ds.V, V, bytes, hipMemcpyHostToDevice, c->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(c->ds.V, U, bytes, hipMemcpyHostToDevice, c->stream));   /* staging */
+        int rc = fused_load_state(c->stream, c->fs, c->dg, c->ds, c->ds.V);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->have_state = true;
+    return CYLMHD_OK;
+}
+
+int cylmhd_set_initial(cylmhd_ctx *c, const double *U, const double *V)
+{
+    if (!c || !U || !V) return CYLMHD_EINVAL;
+    const int NX = c->g.NX, NY = c->g.NY, NZ = c->g.NZ;
+    const size_t N = c->ds.N, NP = (size_t)NX * NY;
+    HIPCHK(hipMemcpy(c->ds.Uinit, U, 9 * N * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> vz(9 * 2 * NP);
+    for (int m = 0; m < 9; m++)
+        for (size_t ij = 0; ij < NP; ij++) {
+            vz[(size_t)m * 2 * NP + ij] = V[(size_t)m * N + ij * NZ + 0];
+            vz[(size_t)m * 2 * NP + NP + ij] = V[(size_t)m * N + ij * NZ + (NZ - 1)];
+        }
+    HIPCHK(hipMemcpy(c->ds.Vinit_z, vz.data(), vz.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (c->p.path != CYLMHD_PATH_PLAIN64) {
+        int rc = fused_set_rhofloor(c->stream, c->fs, c->ds, c->p.rhocr);
+        if (rc) return rc;
+    }
+    return cylmhd_upload_state(c, U, V);
+}
+
+int cylmhd_download_state(cylmhd_ctx *c, double *U, double *V)
+{
+    if (!c || !c->have_state) return CYLMHD_EINVAL;
+    const size_t bytes = 9 * c->ds.N * sizeof(double);
+    if (c->p.path == CYLMHD_PATH_PLAIN64) {
+        if (U) HIPCHK(hipMemcpyAsync(U, c->ds.U, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (V) HIPCHK(hipMemcpyAsync(V, c->ds.V, bytes, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        /* U (FP64 copy of the path's state) -> fs.u64_scratch, V = UtoV(U) -> ds.V */
+        Consts k = make_consts(c, 0.0, 0.0);
+        int rc = fused_export(c->stream, c->fs, c->dg, c->ds, k, c->ds.V, nullptr);
+        if (rc) return rc;
+        if (U) HIPCHK(hipMemcpyAsync(U, c->fs.u64_scratch, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (V) HIPCHK(hipMemcpyAsync(V, c->ds.V, bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CYLMHD_OK;
+}
+
+int cylmhd_setdt(cylmhd_ctx *c, double *dt, double *ch, int ijk[3])
+{
+    if (!c || !c->have_state) return CYLMHD_EINVAL;
+    Consts k = make_consts(c, 0.0, 0.0);
+    double res[2];
+    if (c->p.path == CYLMHD_PATH_PLAIN64) {
+        launch_setdt(c->stream, c->dg, k, c->ds.V, c->ds.red, c->ds.nred);
+    } else {
+        int rc = fused_setdt(c->stream, c->fs, c->dg, c->ds, k);
+        if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(res, c->ds.red, sizeof res, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const double oneocfldt = res[0];
+    const long long ix = res[1] < 0 ? -1 : (long long)res[1];
+    if (dt) *dt = c->p.cfl / oneocfldt;                 /* setdt.h:84 */
+    if (ch) *ch = c->g.minminlength * oneocfldt;        /* setdt.h:89 */
+    if (ijk) {
+        const long long NY = c->g.NY, NZ = c->g.NZ;
+        ijk[0] = ix < 0 ? -1 : (int)(ix / (NY * NZ));
+        ijk[1] = ix < 0 ? -1 : (int)((ix / NZ) % NY);
+        ijk[2] = ix < 0 ? -1 : (int)(ix % NZ);
+    }
+    return CYLMHD_OK;
+}
+
+int cylmhd_gravity(cylmhd_ctx *c, double t)
+{
+    if (!c) return CYLMHD_EINVAL;
+    c->dg.epssp = cyl_spiral_eps(&c->p, t);
+    launch_sincos(c->stream, c->dg, c->d_sincos, c->d_logxs, t, c->p.omegapattern, c->p.omegasp);
+    HIPCHK(hipGetLastError());
+    return CYLMHD_OK;
+}
+
+int cylmhd_step(cylmhd_ctx *c, double dt, double ch)
+{
+    if (!c || !c->have_state) return CYLMHD_EINVAL;
+    const Consts k = make_consts(c, dt, ch);
+    const bool rk3 = c->p.rk != 2;
+    const int nst = rk3 ? 3 : 2;
+    HIPCHK(hipMemsetAsync(c->ds.counts, 0, 9 * sizeof(int), c->stream));
+    for (int s = 1; s <= nst; s++) {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (c->timing && c->ev_used < kEventPairs) {
+            e0 = c->ev[c->ev_used][0];
+            e1 = c->ev[c->ev_used][1];
+            c->ev_used++;
+        }
+        StageArgs a{s, rk3, c->p.limiter == CYLMHD_LIMITER_MC, c->p.solver == CYLMHD_SOLVER_HLL};
+        if (c->p.path == CYLMHD_PATH_PLAIN64) {
+            plain_stage(c->stream, c->dg, c->ds, k, a, e0, e1);
+        } else {
+            int rc = fused_stage(c->stream, c->fs, c->dg, c->ds, k, a, e0, e1);
+            if (rc) return rc;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    return CYLMHD_OK;
+}
+
+int cylmhd_floor_counts(cylmhd_ctx *c, int counts[9])
+{
+    if (!c || !counts) return CYLMHD_EINVAL;
+    HIPCHK(hipMemcpyAsync(counts, c->ds.counts, 9 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CYLMHD_OK;
+}
+
+int cylmhd_diagnostics(cylmhd_ctx *c, cylmhd_diag *d)
+{
+    if (!c || !d || !c->have_state) return CYLMHD_EINVAL;
+    double res[4];
+    if (c->p.path == CYLMHD_PATH_PLAIN64) {
+        launch_diag(c->stream, c->dg, c->ds.U, c->ds.red, c->ds.nred);
+    } else {
+        int rc = fused_diag(c->stream, c->fs, c->dg, c->ds);
+        if (rc) return rc;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(res, c->ds.red, sizeof res, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    d->mass = res[0]; d->energy = res[1]; d->divb_l1 = res[2]; d->psi_l1 = res[3];
+    return CYLMHD_OK;
+}
+
+int cylmhd_sync(cylmhd_ctx *c)
+{
+    if (!c) return CYLMHD_EINVAL;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipGetLastError());
+    return CYLMHD_OK;
+}
+
+int cylmhd_kernel_timing(cylmhd_ctx *c, int which, int reset, double *avg_ms, int *launches)
+{
+    if (!c || which != 0) return CYLMHD_EINVAL;
+    if (!c->ev_ready) {
+        for (int q = 0; q < kEventPairs; q++) {
+            HIPCHK(hipEventCreate(&c->ev[q][0]));
+            HIPCHK(hipEventCreate(&c->ev[q][1]));
+        }
+        c->ev_ready = true;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (int q = 0; q < c->ev_used; q++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[q][0], c->ev[q][1]));
+        tot += ms;
+    }
+    if (avg_ms) *avg_ms = c->ev_used ? tot / c->ev_used : 0.0;
+    if (launches) *launches = c->ev_used;
+    if (reset) c->ev_used = 0;
+    c->timing = true;
+    return CYLMHD_OK;
+}
+
+}  // extern "C"

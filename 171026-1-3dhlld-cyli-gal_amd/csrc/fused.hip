@@ -1,0 +1,815 @@
+/*
+ * fused.hip -- one fused kernel per RK stage (FP64 or FP32 state).
+ *
+ * k_fused marches a (15 x 16) column of (phi, z) cells through r.  At plane
+ * i it holds in LDS the primitives of planes i, i+1, i+2 (a 3-slot ring;
+ * each plane is loaded and converted UtoV once) plus the phi/z halo of plane
+ * i, and
+ *   - x: computes the states of cell i+1 (one limiter call gives both of its
+ *        faces), forms the r-face flux at i+1/2 (the one at i-1/2 is carried
+ *        in registers from the previous plane);
+ *   - y: 16 x 16 phi-faces into an LDS face buffer;
+ *   - z: 15 x 17 z-faces into the same buffer;
+ *   - accumulates the divergence in the reference's order
+ *        ((g2 F+ - g1 F-) + (g4 G+ - g3 G-)) + (g6 H+ - g5 H-),
+ *     adds SpiralGravityCooling, forms the RK stage combination, applies
+ *     the absorbing layers (last stage), Check, and stores U;
+ *   - writes the pre-Check values of the cells the boundary map reads into
+ *     a shadow array (for k_halo), and on the last stage the Setdt partial.
+ * Reads per cell and stage: U_in once (plus 1.5x halo, mostly L2 hits), U^n
+ * once on stages 2-3; writes U_out once.  V, the face fluxes and S never
+ * touch HBM.
+ *
+ * Against the plain path the only change in the arithmetic is that V is
+ * recomputed from the stored U each stage instead of being carried from
+ * the previous Check (in floored cells the two differ in the last bits).
+ * FP32 state keeps the pressure recovery of UtoV, the rebuild of the energy
+ * in Check, the CFL reduction and the diagnostics in FP64.
+ *
+ * Boundary faces: phi is periodic in both the cell and the interface
+ * boundaries (YVlrPeriodic), so halo rows wrap (-1 -> NY-3, NY -> 2); the r
+ * faces 0 and NX-2 both take (upper(NX-2), lower(1)) (XVlrPeriodic) and are
+ * computed by k_edge; the z faces 0 and NZ-2 follow ZLeftVlVzAntisymBzsym /
+ * ZRightVrFree (minmod) or ZVlrFixed (MC).
+ */
+#include "fused.h"
+
+#include <algorithm>
+
+namespace cyl {
+
+#define C3(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * g.NZ + (size_t)(k))
+
+constexpr int TJ = 15, TK = 16, NT = 256, NCELL = TJ * TK, NHALO = 4 * TK + 4 * TJ;
+
+/* ---- precision-specific UtoV and Check ---------------------------------- */
+template <typename T> struct Prec;
+
+template <> struct Prec<double> {
+    static __device__ __forceinline__ void utov(const Consts &c, const double *U, double oneox, double *V)
+    {
+        cyl::utov<double>(c, U, oneox, V);
+    }
+    static __device__ __forceinline__ int check(const Consts &c, double *U, double *V, double x, double rf)
+    {
+        return cyl::check<double>(c, U, V, x, rf);
+    }
+};
+
+/* FP32 state: p = gm1 (E - (rho v)^2/2rho - B^2/2) and the energy rebuild
+ * of Check are formed in FP64 (the cancellation points of low-beta and
+ * high-Mach cells); the rest stays FP32. */
+template <> struct Prec<float> {
+    static __device__ __forceinline__ void utov(const Consts &c, const float *U, double oneox, float *V)
+    {
+        const float oneorho = 1.0f / U[0];
+        const double d1 = U[1], d2 = U[2], d3 = U[3], d4 = U[4], d5 = U[5], d6 = U[6];
+        const double b2 = d4 * d4 + d5 * d5 + d6 * d6;
+        const double r2v2 = d1 * d1 + d2 * d2 * oneox * oneox + d3 * d3;
+        V[0] = U[0];
+        V[1] = U[1] * oneorho;
+        V[2] = U[2] * oneorho * (float)oneox;
+        V[3] = U[3] * oneorho;
+        V[4] = U[4];
+        V[5] = U[5];
+        V[6] = U[6];
+        V[7] = (float)(c.gm1 * ((double)U[7] - 0.5 * (r2v2 / (double)U[0] + b2)));
+        V[8] = U[8];
+    }
+    static __device__ __forceinline__ int check(const Consts &c, float *U, float *V, double x, float rf)
+    {
+        int flags = 0;
+        const double v2 = (double)V[2] * V[2];
+        const double b2 = ((double)V[4] * V[4] + (double)V[5] * V[5] + (double)V[6] * V[6]);
+        if (V[0] < rf) {
+            flags |= 1;
+            V[0] = rf; V[1] = 0.0f; V[3] = 0.0f;
+            U[0] = V[0]; U[1] = 0.0f; U[2] = (float)(x * V[0] * V[2]); U[3] = 0.0f;
+            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7]);
+        }
+        const float prfloor = V[0] * (float)c.prcr;
+        if (V[7] < prfloor) {
+            flags |= 2;
+            V[1] = 0.0f; V[3] = 0.0f; V[7] = prfloor;
+            U[1] = 0.0f; U[3] = 0.0f;
+            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7]);
+        }
+        const double oneobeta = 0.5 * b2 / (double)V[7];
+        if (oneobeta > c.oneobetacr) {
+            flags |= 4;
+            V[0] = (float)(c.betacr * oneobeta * V[0]); V[1] = 0.0f; V[3] = 0.0f;
+            V[7] = (float)(c.betacr * oneobeta * V[7]);
+            U[0] = V[0]; U[1] = 0.0f; U[2] = (float)(x * V[0] * V[2]); U[3] = 0.0f;
+            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7]);
+        }
+        return flags;
+    }
+};
+
+template <typename T> struct FArgs {
+    DevGeom g;
+    Consts c;
+    const T *Uin, *Un;
+    T *Uout, *shadow;
+    const T *rhofloor;
+    const double *Uinit, *Vinit_z;
+    const T *edge;
+    int *counts;
+    double *dtpart;
+    size_t N;
+    int kind, seglen, tiles_j, tiles_k, ntiles, nblocks;
+    bool last, rk3, want_dt;
+};
+
+__device__ __forceinline__ int jmap(const DevGeom &g, int jg)
+{
+    if (jg == -1) return g.NY - 3;                  /* periodic wrap of the phi halo */
+    if (jg == g.NY) return 2;
+    return jg < 0 ? 0 : (jg > g.NY - 1 ? g.NY - 1 : jg);
+}
+__device__ __forceinline__ int kmap(const DevGeom &g, int kg) { return kg < 0 ? 0 : (kg > g.NZ - 1 ? g.NZ - 1 : kg); }
+
+/* stage input of one cell as primitives: psi damped (GLMSourceSolver) then UtoV */
+template <typename T>
+__device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, T *v)
+{
+    T u[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) u[m] = a.Uin[m * a.N + cc];
+    u[8] = T(a.c.psicoef) * u[8];
+    Prec<T>::utov(a.c, u, a.g.oneox[i], v);
+}
+
+/* XCD-aware block order: consecutive tiles (which share halo planes) go to
+ * the same XCD's L2 (blocks are dealt round-robin over the 8 XCDs).  The
+ * grid is a multiple of 8, so this is a bijection of [0, gridDim.x). */
+__device__ __forceinline__ int remap_block(int b, int nb)
+{
+    const int per = nb / 8;
+    return (b % 8) * per + b / 8;
+}
+
+template <typename T, bool MC, bool HLL>
+__global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
+{
+    const DevGeom &g = a.g;
+    const Consts &c = a.c;
+    __shared__ T ring[3][9][NCELL];
+    __shared__ T hal[9][NHALO];
+    __shared__ T fbuf[9][NT];
+
+    const int t = threadIdx.x;
+    const int b = remap_block(blockIdx.x, gridDim.x);
+    const int tile = b % a.ntiles, seg = b / a.ntiles;
+    const int tj = tile / a.tiles_k, tk = tile % a.tiles_k;
+    const int j0 = 1 + tj * TJ, k0 = 1 + tk * TK;
+    const int ibeg = 1 + seg * a.seglen;
+    const int iend = min(g.NX - 2, ibeg + a.seglen - 1);
+    if (b >= a.nblocks || ibeg > iend) {          /* surplus block: neutral Setdt partial */
+        if (a.want_dt && t == 0) { a.dtpart[2 * blockIdx.x] = 0.0; a.dtpart[2 * blockIdx.x + 1] = 9.3e18; }
+        return;
+    }
+    const bool is_cell = t < NCELL;
+    const int cj = t / TK, ck = t % TK;                 /* cell coordinates in the tile */
+    const int gj = j0 + cj, gk = k0 + ck;
+    const bool valid = is_cell && gj <= g.NY - 2 && gk <= g.NZ - 2;
+    const size_t N = a.N;
+
+    auto load_center = [&](int p) {
+        if (is_cell && p >= 0 && p <= g.NX - 1) {
+            T v[9];
+            cell_V(a, C3(p, jmap(g, gj), kmap(g, gk)), p, v);
+#pragma unroll
+            for (int m = 0; m < 9; m++) ring[p % 3][m][t] = v[m];
+        }
+    };
+    auto load_halo = [&](int p) {
+        if (t < NHALO) {
+            int jj, kk;
+            if (t < 2 * TK) { jj = -2 + t / TK; kk = t % TK; }
+            else if (t < 4 * TK) { jj = TJ + (t - 2 * TK) / TK; kk = (t - 2 * TK) % TK; }
+            else if (t < 4 * TK + 2 * TJ) { const int u = t - 4 * TK; jj = u / 2; kk = -2 + u % 2; }
+            else { const int u = t - 4 * TK - 2 * TJ; jj = u / 2; kk = TK + u % 2; }
+            T v[9];
+            cell_V(a, C3(p, jmap(g, j0 + jj), kmap(g, k0 + kk)), p, v);
+#pragma unroll
+            for (int m = 0; m < 9; m++) hal[m][t] = v[m];
+        }
+    };
+    /* V of plane in ring slot s at tile coords (jj, kk), halo included */
+    auto getv = [&](int s, int jj, int kk, T *v) {
+        int idx;
+        if (jj < 0) idx = (jj + 2) * TK + kk;
+        else if (jj >= TJ) idx = 2 * TK + (jj - TJ) * TK + kk;
+        else if (kk < 0) idx = 4 * TK + jj * 2 + (kk + 2);
+        else if (kk >= TK) idx = 4 * TK + 2 * TJ + jj * 2 + (kk - TK);
+        else idx = -1;
+        if (idx >= 0) {
+#pragma unroll
+            for (int m = 0; m < 9; m++) v[m] = hal[m][idx];
+        } else {
+#pragma unroll
+            for (int m = 0; m < 9; m++) v[m] = ring[s][m][jj * TK + kk];
+        }
+    };
+
+    T up_prev[9], fx_prev[9], acc[9];
+    int f0 = 0, f1 = 0, f2 = 0;
+    double best = 0.0;
+    long long bix = LLONG_MAX;
+    const size_t NE = (size_t)g.NY * g.NZ;
+
+    /* ---- prologue: states of cell ibeg and the flux at ibeg - 1/2 -------- */
+    if (ibeg == 1) {
+        load_center(0); load_center(1); load_center(2);
+        __syncthreads();
+        if (is_cell) {
+            T vm[9], v0[9], vp[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) { vm[m] = ring[0][m][t]; v0[m] = ring[1][m][t]; vp[m] = ring[2][m][t]; }
+            cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[1]), T(g.oneodx[0]), T(1.0), up_prev, (T *)nullptr);
+            const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
+#pragma unroll
+            for (int m = 0; m < 9; m++) fx_prev[m] = a.edge[m * NE + e];   /* face 0 */
+        }
+    } else {
+        T upm[9];
+        load_center(ibeg - 2); load_center(ibeg - 1); load_center(ibeg);
+        __syncthreads();
+        if (is_cell) {
+            T vm[9], v0[9], vp[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) {
+                vm[m] = ring[(ibeg - 2) % 3][m][t]; v0[m] = ring[(ibeg - 1) % 3][m][t]; vp[m] = ring[ibeg % 3][m][t];
+            }
+            cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[ibeg - 1]), T(g.oneodx[ibeg - 2]), T(1.0), upm, (T *)nullptr);
+        }
+        __syncthreads();
+        load_center(ibeg + 1);
+        __syncthreads();
+        if (is_cell) {
+            T vm[9], v0[9], vp[9], lo[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) {
+                vm[m] = ring[(ibeg - 1) % 3][m][t]; v0[m] = ring[ibeg % 3][m][t]; vp[m] = ring[(ibeg + 1) % 3][m][t];
+            }
+            cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[ibeg]), T(g.oneodx[ibeg - 1]), T(1.0), up_prev, lo);
+            riemann<0, HLL>(c, upm, lo, T(g.xm[ibeg - 1]), T(g.oneox[ibeg - 1]), fx_prev);
+        }
+    }
+    if (ibeg == 1) {
+        /* ring holds 0,1,2; the loop below expects i, i+1 loaded: fine */
+    } else {
+        /* ring holds ibeg-1, ibeg, ibeg+1 */
+    }
+
+    for (int i = ibeg; i <= iend; i++) {
+        __syncthreads();
+        if (i + 2 <= g.NX - 1 && i < g.NX - 2) load_center(i + 2);
+        load_halo(i);
+        __syncthreads();
+        const int si = i % 3;
+        /* ---- x: flux at i+1/2 ------------------------------------------- */
+        T fxn[9];
+        if (is_cell) {
+            if (i == g.NX - 2) {
+                const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
+#pragma unroll
+                for (int m = 0; m < 9; m++) fxn[m] = a.edge[9 * NE + m * NE + e];   /* face NX-2 */
+            } else {
+                T vm[9], v0[9], vp[9], lo[9], up[9];
+#pragma unroll
+                for (int m = 0; m < 9; m++) { vm[m] = ring[si][m][t]; v0[m] = ring[(i + 1) % 3][m][t]; vp[m] = ring[(i + 2) % 3][m][t]; }
+                cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[i + 1]), T(g.oneodx[i]), T(1.0), up, lo);
+                riemann<0, HLL>(c, up_prev, lo, T(g.xm[i]), T(g.oneox[i]), fxn);
+#pragma unroll
+                for (int m = 0; m < 9; m++) up_prev[m] = up[m];
+            }
+            const size_t ik = (size_t)i * g.NZ + kmap(g, gk);
+            const T g1 = T(g.g1[ik]), g2 = T(g.g2[ik]);
+#pragma unroll
+            for (int m = 0; m < 9; m++) { acc[m] = g2 * fxn[m] - g1 * fx_prev[m]; fx_prev[m] = fxn[m]; }
+        }
+        /* ---- y: 16 x 16 faces ------------------------------------------- */
+        {
+            const int fj = t / TK, fk = t % TK;          /* face between rows fj-1 and fj */
+            T vm[9], v0[9], vp[9], vl[9], vr[9], fy[9];
+            const T f1 = T(g.oneox[i]), f2 = T(g.oneody);
+            getv(si, fj - 2, fk, vm); getv(si, fj - 1, fk, v0); getv(si, fj, fk, vp);
+            cell_states<1, MC>(c, vm, v0, vp, f1, f1, f2, vl, (T *)nullptr);
+            getv(si, fj + 1, fk, vm);       /* reuse: rows fj-1, fj, fj+1 */
+            cell_states<1, MC>(c, v0, vp, vm, f1, f1, f2, (T *)nullptr, vr);
+            riemann<1, HLL>(c, vl, vr, T(g.x[i]), T(g.oneox[i]), fy);
+#pragma unroll
+            for (int m = 0; m < 9; m++) fbuf[m][t] = fy[m];
+        }
+        __syncthreads();
+        if (is_cell) {
+            const size_t ik = (size_t)i * g.NZ + kmap(g, gk);
+            const T g3 = T(g.g3[ik]), g4 = g3;
+#pragma unroll
+            for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g4 * fbuf[m][(cj + 1) * TK + ck] - g3 * fbuf[m][cj * TK + ck]);
+        }
+        __syncthreads();
+        /* ---- z: 15 x 17 faces ------------------------------------------- */
+        if (t < TJ * (TK + 1)) {
+            const int fj = t / (TK + 1), fk = t % (TK + 1);   /* face between cols fk-1 and fk */
+            const int kf = k0 + fk - 1;                       /* global face index */
+            const int jgl = jmap(g, j0 + fj);
+            T vm[9], v0[9], vp[9], vl[9], vr[9], fz[9];
+            if (kf == 0) {
+                if (MC) {
+                    const size_t NP = (size_t)g.NX * g.NY;
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vl[m] = T(a.Vinit_z[(size_t)m * 2 * NP + (size_t)i * g.NY + jgl]);
+                } else {
+                    /* lower state of cell 2 (col fk+1), vz negated */
+                    getv(si, fj, fk, vm); getv(si, fj, fk + 1, v0); getv(si, fj, fk + 2, vp);
+                    cell_states<2, MC>(c, vm, v0, vp, T(g.oneodz[2]), T(g.oneodz[1]), T(1.0), (T *)nullptr, vl);
+                    vl[3] = -vl[3];
+                }
+            } else {
+                getv(si, fj, fk - 2, vm); getv(si, fj, fk - 1, v0); getv(si, fj, fk, vp);
+                const int kc = kmap(g, kf);
+                cell_states<2, MC>(c, vm, v0, vp, T(g.oneodz[kc]), T(g.oneodz[kc > 0 ? kc - 1 : 0]), T(1.0), vl, (T *)nullptr);
+            }
+            if (kf == g.NZ - 2) {
+                if (MC) {
+                    const size_t NP = (size_t)g.NX * g.NY;
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vr[m] = T(a.Vinit_z[(size_t)m * 2 * NP + NP + (size_t)i * g.NY + jgl]);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vr[m] = vl[m];
+                }
+            } else {
+                getv(si, fj, fk - 1, vm); getv(si, fj, fk, v0); getv(si, fj, fk + 1, vp);
+                const int kc = kmap(g, kf + 1);
+                cell_states<2, MC>(c, vm, v0, vp, T(g.oneodz[kc]), T(g.oneodz[kc > 0 ? kc - 1 : 0]), T(1.0), (T *)nullptr, vr);
+            }
+            riemann<2, HLL>(c, vl, vr, T(g.x[i]), T(g.oneox[i]), fz);
+#pragma unroll
+            for (int m = 0; m < 9; m++) fbuf[m][t] = fz[m];
+        }
+        __syncthreads();
+        /* ---- divergence, sources, stage update, absorb, Check, store ----- */
+        if (valid) {
+            const size_t ik = (size_t)i * g.NZ + gk;
+            const T g5 = T(g.g5[ik]), g6 = g5;
+#pragma unroll
+            for (int m = 0; m < 9; m++)
+                acc[m] = acc[m] + (g6 * fbuf[m][cj * (TK + 1) + ck + 1] - g5 * fbuf[m][cj * (TK + 1) + ck]);
+            T v[9], S[9], u[9], un[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) v[m] = ring[si][m][t];
+            double gx, gy, gz;
+            const bool spiral = gravity_cell(g, i, gj, gk, gx, gy, gz);
+            sources(c, v, T(g.x[i]), T(g.oneox[i]), T(gx), T(gy), T(gz), spiral, S);
+            const size_t cc = C3(i, gj, gk);
+#pragma unroll
+            for (int m = 0; m < 9; m++) u[m] = a.Uin[m * N + cc];
+            u[8] = T(c.psicoef) * u[8];
+            const int kind = a.kind;
+            if (kind == 2 || kind == 3 || kind == 5) {
+#pragma unroll
+                for (int m = 0; m < 9; m++) un[m] = a.Un[m * N + cc];
+                un[8] = T(c.psicoef) * un[8];
+            }
+            const T dt = T(c.dt), oneo3 = T(1.0) / T(3.0);
+#pragma unroll
+            for (int m = 0; m < 9; m++) {
+                const T div = acc[m];
+                if (kind == 1 || kind == 4) u[m] = u[m] - dt * div + dt * S[m];
+                else if (kind == 2) u[m] = T(0.25) * (T(3.0) * un[m] + u[m] - dt * div + dt * S[m]);
+                else if (kind == 3) u[m] = oneo3 * (un[m] + T(2.0) * (u[m] - dt * div + dt * S[m]));
+                else u[m] = T(0.5) * (un[m] + u[m] - dt * div + dt * S[m]);
+            }
+            if (!a.rk3 && gk == 1) u[3] = T(0.0);       /* ZLeftVzAntisymBSym, U3[k=1] */
+            if (i == 1 || i == g.NX - 2 || gj == 1 || gj == g.NY - 2 || gk <= 2 || gk == g.NZ - 2) {
+#pragma unroll
+                for (int m = 0; m < 9; m++) a.shadow[m * N + cc] = u[m];
+            }
+            if (a.last) {
+                const bool ax = i >= g.NX - 11, ac = i <= g.irin, az = !a.rk3 && gk >= g.NZ - 11;
+                if (ax || ac || az) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) {
+                        const T ui = T(a.Uinit[m * N + cc]);
+                        if (ax) u[m] -= (T(g.xabs[i - (g.NX - 11)]) * (u[m] - ui));
+                        if (ac) u[m] -= (T(g.cabs[(size_t)i * g.NZ + gk]) * (u[m] - ui));
+                        if (az) u[m] -= (T(g.zabs[gk - (g.NZ - 11)]) * (u[m] - ui));
+                    }
+                }
+            }
+            Prec<T>::utov(c, u, g.oneox[i], v);
+            const int fl = Prec<T>::check(c, u, v, g.x[i], a.rhofloor[cc]);
+            f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
+#pragma unroll
+            for (int m = 0; m < 9; m++) a.Uout[m * N + cc] = u[m];
+            if (a.want_dt && i < g.NX - 11) {
+                /* Setdt (setdt.h:33-69) on the checked primitives, FP64 */
+                const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
+                const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
+                const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
+                const double t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / g.minlength[ik];
+                const double tc = (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7)));
+                const double tt = rmax(t1, tc);
+                const long long ix = (long long)cc;
+                if ((tt > best || (tt == best && ix < bix)) && tt > 0.0) { best = tt; bix = ix; }
+            }
+        }
+    }
+    /* ---- floor counters: one atomic per wave ---------------------------- */
+    for (int off = 32; off > 0; off >>= 1) {
+        f0 += __shfl_xor(f0, off); f1 += __shfl_xor(f1, off); f2 += __shfl_xor(f2, off);
+    }
+    if ((t & 63) == 0) {
+        if (f0) atomicAdd(&a.counts[0], f0);
+        if (f1) atomicAdd(&a.counts[1], f1);
+        if (f2) atomicAdd(&a.counts[2], f2);
+    }
+    if (a.want_dt) {
+        __syncthreads();
+        double *sv = (double *)&fbuf[0][0];
+        long long *sx = (long long *)(sv + NT);
+        sv[t] = best; sx[t] = bix;
+        __syncthreads();
+        for (int off = NT / 2; off > 0; off >>= 1) {
+            if (t < off) {
+                const double v = sv[t + off];
+                const long long ix = sx[t + off];
+                if (v > sv[t] || (v == sv[t] && ix < sx[t])) { sv[t] = v; sx[t] = ix; }
+            }
+            __syncthreads();
+        }
+        if (t == 0) { a.dtpart[2 * blockIdx.x] = sv[0]; a.dtpart[2 * blockIdx.x + 1] = (double)sx[0]; }
+    }
+}
+
+/* ---- r-boundary faces 0 and NX-2: (upper(NX-2), lower(1)) ---------------- */
+template <typename T, bool MC, bool HLL>
+__global__ __launch_bounds__(256) void k_edge(FArgs<T> a, T *edge)
+{
+    const DevGeom &g = a.g;
+    const int nj = g.NY - 2, nk = g.NZ - 2;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nj * nk) return;
+    const int j = 1 + q / nk, k = 1 + q % nk;
+    T vm[9], v0[9], vp[9], up[9], lo[9], f[9];
+    const int L = g.NX - 2;
+    cell_V(a, C3(L - 1, j, k), L - 1, vm); cell_V(a, C3(L, j, k), L, v0); cell_V(a, C3(L + 1, j, k), L + 1, vp);
+    cell_states<0, MC>(a.c, vm, v0, vp, T(g.oneodx[L]), T(g.oneodx[L - 1]), T(1.0), up, (T *)nullptr);
+    cell_V(a, C3(0, j, k), 0, vm); cell_V(a, C3(1, j, k), 1, v0); cell_V(a, C3(2, j, k), 2, vp);
+    cell_states<0, MC>(a.c, vm, v0, vp, T(g.oneodx[1]), T(g.oneodx[0]), T(1.0), (T *)nullptr, lo);
+    const size_t NE = (size_t)g.NY * g.NZ, e = (size_t)j * g.NZ + k;
+    riemann<0, HLL>(a.c, up, lo, T(g.xm[0]), T(g.oneox[0]), f);
+#pragma unroll
+    for (int m = 0; m < 9; m++) edge[m * NE + e] = f[m];
+    riemann<0, HLL>(a.c, up, lo, T(g.xm[L]), T(g.oneox[L]), f);
+#pragma unroll
+    for (int m = 0; m < 9; m++) edge[9 * NE + m * NE + e] = f[m];
+}
+
+/* ---- ghost cells: boundary map of the shadow, then absorb / UtoV / Check */
+template <typename T>
+__global__ __launch_bounds__(256) void k_halo(FArgs<T> a)
+{
+    const DevGeom &g = a.g;
+    const size_t NX = g.NX, NY = g.NY, NZ = g.NZ, N = a.N;
+    const size_t nA = 2 * NY * NZ, nB = (NX - 2) * 2 * NZ, nC = (NX - 2) * (NY - 2) * 2;
+    const size_t tot = nA + nB + nC;
+    int f0 = 0, f1 = 0, f2 = 0;
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+        int i, j, k;
+        size_t r = t;
+        if (r < nA) { i = (r / (NY * NZ)) ? (int)NX - 1 : 0; r %= NY * NZ; j = (int)(r / NZ); k = (int)(r % NZ); }
+        else if ((r -= nA) < nB) { i = 1 + (int)(r / (2 * NZ)); r %= 2 * NZ; j = (r / NZ) ? (int)NY - 1 : 0; k = (int)(r % NZ); }
+        else { r -= nB; i = 1 + (int)(r / ((NY - 2) * 2)); r %= (NY - 2) * 2; j = 1 + (int)(r / 2); k = (r % 2) ? (int)NZ - 1 : 0; }
+        int si, sj, sk;
+        unsigned sign;
+        bool zero3;
+        ghost_source(g, a.rk3, i, j, k, si, sj, sk, sign, zero3);
+        const size_t src = C3(si, sj, sk), dst = C3(i, j, k);
+        T u[9], v[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) {
+            const T val = a.shadow[m * N + src];
+            u[m] = (sign >> m) & 1u ? -val : val;
+        }
+        if (zero3) u[3] = T(0.0);
+        if (a.last) {
+            const bool ax = i >= g.NX - 11, ac = i <= g.irin, az = !a.rk3 && k >= g.NZ - 11;
+            if (ax || ac || az) {
+#pragma unroll
+                for (int m = 0; m < 9; m++) {
+                    const T ui = T(a.Uinit[m * N + dst]);
+                    if (ax) u[m] -= (T(g.xabs[i - (g.NX - 11)]) * (u[m] - ui));
+                    if (ac) u[m] -= (T(g.cabs[(size_t)i * g.NZ + k]) * (u[m] - ui));
+                    if (az) u[m] -= (T(g.zabs[k - (g.NZ - 11)]) * (u[m] - ui));
+                }
+            }
+        }
+        Prec<T>::utov(a.c, u, g.oneox[i], v);
+        const int fl = Prec<T>::check(a.c, u, v, g.x[i], a.rhofloor[dst]);
+        f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
+#pragma unroll
+        for (int m = 0; m < 9; m++) a.Uout[m * N + dst] = u[m];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        f0 += __shfl_xor(f0, off); f1 += __shfl_xor(f1, off); f2 += __shfl_xor(f2, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (f0) atomicAdd(&a.counts[0], f0);
+        if (f1) atomicAdd(&a.counts[1], f1);
+        if (f2) atomicAdd(&a.counts[2], f2);
+    }
+}
+
+/* ---- RK2 only: interior plane k = 1 gets U3 = 0 after the update (the
+ * fused kernel does it for its cells; nothing else to do) ---------------- */
+
+/* ---- Setdt from the stored state (first step, or after an upload) ------- */
+template <typename T>
+__global__ __launch_bounds__(256) void k_setdt_state(FArgs<T> a)
+{
+    const DevGeom &g = a.g;
+    const size_t ni = (size_t)(g.NX - 12), nj = (size_t)(g.NY - 2), nk = (size_t)(g.NZ - 2);
+    const size_t M = ni * nj * nk;
+    double best = 0.0;
+    long long bix = LLONG_MAX;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < M; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(q / (nj * nk)), j = 1 + (int)((q / nk) % nj), k = 1 + (int)(q % nk);
+        const size_t cc = C3(i, j, k);
+        T u[9], v[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) u[m] = a.Un[m * a.N + cc];
+        Prec<T>::utov(a.c, u, g.oneox[i], v);
+        const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
+        const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
+        const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
+        const double t1 = sqrt(v2 + (a.c.gm * V7 + b2) / V0) / g.minlength[(size_t)i * g.NZ + k];
+        const double tc = (2.0 * ((V0 * V0 * cooling(a.c, V0, V7)) / (1.5 * V7)));
+        const double tt = rmax(t1, tc);
+        if ((tt > best || (tt == best && (long long)cc < bix)) && tt > 0.0) { best = tt; bix = (long long)cc; }
+    }
+    __shared__ double sv[256];
+    __shared__ long long sx[256];
+    const int t = threadIdx.x;
+    sv[t] = best; sx[t] = bix;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off && (sv[t + off] > sv[t] || (sv[t + off] == sv[t] && sx[t + off] < sx[t]))) { sv[t] = sv[t + off]; sx[t] = sx[t + off]; }
+        __syncthreads();
+    }
+    if (t == 0) { a.dtpart[2 * blockIdx.x] = sv[0]; a.dtpart[2 * blockIdx.x + 1] = (double)sx[0]; }
+}
+
+__global__ __launch_bounds__(256) void k_dt_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    __shared__ double sv[256];
+    __shared__ long long sx[256];
+    const int t = threadIdx.x;
+    double best = 0.0;
+    long long bix = LLONG_MAX;
+    for (int b = t; b < nb; b += 256) {
+        const double v = part[2 * b];
+        const long long ix = part[2 * b + 1] >= 9.0e18 ? LLONG_MAX : (long long)part[2 * b + 1];
+        if ((v > best || (v == best && ix < bix)) && v > 0.0) { best = v; bix = ix; }
+    }
+    sv[t] = best; sx[t] = bix;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off && (sv[t + off] > sv[t] || (sv[t + off] == sv[t] && sx[t + off] < sx[t]))) { sv[t] = sv[t + off]; sx[t] = sx[t + off]; }
+        __syncthreads();
+    }
+    if (t == 0) { out[0] = sv[0]; out[1] = sx[0] == LLONG_MAX ? -1.0 : (double)sx[0]; }
+}
+
+/* ---- conversions and diagnostics ---------------------------------------- */
+template <typename T>
+__global__ void k_to_state(const double *__restrict__ src, T *__restrict__ dst, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) dst[q] = T(src[q]);
+}
+
+template <typename T>
+__global__ void k_rhofloor(const double *__restrict__ Uinit0, T *__restrict__ rf, size_t n, double rhocr)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) rf[q] = T(Uinit0[q] * rhocr);
+}
+
+template <typename T>
+__global__ void k_export(DevGeom g, Consts c, const T *__restrict__ U, double *__restrict__ U64, double *__restrict__ V64, size_t N)
+{
+    const size_t NYZ = (size_t)g.NY * g.NZ;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < N; q += (size_t)gridDim.x * blockDim.x) {
+        double u[9], v[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) { u[m] = (double)U[m * N + q]; U64[m * N + q] = u[m]; }
+        utov<double>(c, u, g.oneox[q / NYZ], v);
+#pragma unroll
+        for (int m = 0; m < 9; m++) V64[m * N + q] = v[m];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_t(DevGeom g, const T *__restrict__ U, size_t N, double *__restrict__ red)
+{
+    const size_t ni = g.NX - 2, nj = g.NY - 2, nk = g.NZ - 2, M = ni * nj * nk;
+    const size_t si = (size_t)g.NY * g.NZ, sj = g.NZ;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < M; q += (size_t)gridDim.x * blockDim.x) {
+        const int i = 1 + (int)(q / (nj * nk)), j = 1 + (int)((q / nk) % nj), k = 1 + (int)(q % nk);
+        const size_t cc = C3(i, j, k), ik = (size_t)i * g.NZ + k;
+        const double dv = g.dvol[ik];
+        const double b4 = U[4 * N + cc], b5 = U[5 * N + cc], b6 = U[6 * N + cc];
+        const double brp = 0.5 * (b4 + (double)U[4 * N + cc + si]), brm = 0.5 * (b4 + (double)U[4 * N + cc - si]);
+        const double bpp = 0.5 * (b5 + (double)U[5 * N + cc + sj]), bpm = 0.5 * (b5 + (double)U[5 * N + cc - sj]);
+        const double bzp = 0.5 * (b6 + (double)U[6 * N + cc + 1]), bzm = 0.5 * (b6 + (double)U[6 * N + cc - 1]);
+        const double db = (g.g2[ik] * brp - g.g1[ik] * brm) + (g.g3[ik] * bpp - g.g3[ik] * bpm) + (g.g5[ik] * bzp - g.g5[ik] * bzm);
+        acc[0] += (double)U[cc] * dv;
+        acc[1] += (double)U[7 * N + cc] * dv;
+        acc[2] += fabs(db) * dv;
+        acc[3] += fabs((double)U[8 * N + cc]) * dv;
+    }
+    __shared__ double sh[4][256];
+    const int t = threadIdx.x;
+    for (int q = 0; q < 4; q++) sh[q][t] = acc[q];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off)
+            for (int q = 0; q < 4; q++) sh[q][t] += sh[q][t + off];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < 4; q++) red[4 * blockIdx.x + q] = sh[q][0];
+}
+
+__global__ __launch_bounds__(256) void k_diag_final(double *__restrict__ red, int nb)
+{
+    __shared__ double sh[4][256];
+    const int t = threadIdx.x;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = t; b < nb; b += 256)
+        for (int q = 0; q < 4; q++) acc[q] += red[4 * b + q];
+    for (int q = 0; q < 4; q++) sh[q][t] = acc[q];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off)
+            for (int q = 0; q < 4; q++) sh[q][t] += sh[q][t + off];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < 4; q++) red[q] = sh[q][0];
+}
+
+/* ======================================================================== */
+static unsigned nblocks(size_t n, unsigned bs, unsigned cap)
+{
+    size_t b = (n + bs - 1) / bs;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(b, cap));
+}
+
+int fused_alloc(FusedState &fs, const DevGeom &g, const DevState &d, bool fp32, AllocFn alloc)
+{
+    fs.fp32 = fp32;
+    fs.esize = fp32 ? 4 : 8;
+    const size_t N = d.N;
+    int rc;
+    if ((rc = alloc(9 * N * fs.esize, &fs.Un)) || (rc = alloc(9 * N * fs.esize, &fs.A)) ||
+        (rc = alloc(9 * N * fs.esize, &fs.B)) || (rc = alloc(9 * N * fs.esize, &fs.shadow)) ||
+        (rc = alloc(N * fs.esize, &fs.rhofloor)) || (rc = alloc(2 * 9 * (size_t)g.NY * g.NZ * fs.esize, &fs.edge)))
+        return rc;
+    void *p;
+    if ((rc = alloc(9 * N * sizeof(double), &p))) return rc;
+    fs.u64_scratch = (double *)p;
+    fs.tiles_j = (g.NY - 2 + TJ - 1) / TJ;
+    fs.tiles_k = (g.NZ - 2 + TK - 1) / TK;
+    const int cols = fs.tiles_j * fs.tiles_k;
+    /* enough blocks to fill 256 CUs several times over, segments >= 8 planes */
+    int nseg = (2048 + cols - 1) / cols;
+    nseg = std::max(1, std::min(nseg, (g.NX - 2) / 8));
+    fs.seglen = (g.NX - 2 + nseg - 1) / nseg;
+    fs.nseg = (g.NX - 2 + fs.seglen - 1) / fs.seglen;
+    fs.dt_blocks = std::max((cols * fs.nseg + 7) / 8 * 8, 1024);
+    if ((rc = alloc(2 * (size_t)fs.dt_blocks * sizeof(double), &p))) return rc;
+    fs.dtpart = (double *)p;
+    return 0;
+}
+
+template <typename T>
+static FArgs<T> make_args(FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c)
+{
+    FArgs<T> a{};
+    a.g = g; a.c = c;
+    a.N = d.N;
+    a.rhofloor = (const T *)fs.rhofloor;
+    a.Uinit = d.Uinit; a.Vinit_z = d.Vinit_z;
+    a.edge = (const T *)fs.edge;
+    a.counts = d.counts;
+    a.dtpart = fs.dtpart;
+    a.seglen = fs.seglen; a.tiles_j = fs.tiles_j; a.tiles_k = fs.tiles_k; a.ntiles = fs.tiles_j * fs.tiles_k;
+    a.Un = (const T *)fs.Un;
+    a.shadow = (T *)fs.shadow;
+    return a;
+}
+
+int fused_set_rhofloor(hipStream_t s, FusedState &fs, const DevState &d, double rhocr)
+{
+    if (fs.fp32) hipLaunchKernelGGL(k_rhofloor<float>, dim3(nblocks(d.N, 256, 8192)), dim3(256), 0, s, d.Uinit, (float *)fs.rhofloor, d.N, rhocr);
+    else hipLaunchKernelGGL(k_rhofloor<double>, dim3(nblocks(d.N, 256, 8192)), dim3(256), 0, s, d.Uinit, (double *)fs.rhofloor, d.N, rhocr);
+    return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
+}
+
+int fused_load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const double *U64)
+{
+    (void)g;
+    const size_t n = 9 * d.N;
+    if (fs.fp32) hipLaunchKernelGGL(k_to_state<float>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, U64, (float *)fs.Un, n);
+    else hipLaunchKernelGGL(k_to_state<double>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, U64, (double *)fs.Un, n);
+    fs.dt_valid = false;
+    return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
+}
+
+int fused_export(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c, double *V64,
+                 const double *)
+{
+    if (fs.fp32) hipLaunchKernelGGL(k_export<float>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, c, (const float *)fs.Un, fs.u64_scratch, V64, d.N);
+    else hipLaunchKernelGGL(k_export<double>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, c, (const double *)fs.Un, fs.u64_scratch, V64, d.N);
+    return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
+}
+
+template <typename T>
+static void setdt_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c)
+{
+    int nb = fs.dt_blocks;
+    if (!fs.dt_valid) {
+        FArgs<T> a = make_args<T>(fs, g, d, c);
+        const size_t M = (size_t)(g.NX - 12) * (g.NY - 2) * (g.NZ - 2);
+        nb = (int)nblocks(M, 256, (unsigned)std::min(fs.dt_blocks, 4096));
+        hipLaunchKernelGGL(k_setdt_state<T>, dim3(nb), dim3(256), 0, s, a);
+    } else {
+        nb = (fs.tiles_j * fs.tiles_k * fs.nseg + 7) / 8 * 8;
+    }
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, fs.dtpart, nb, d.red);
+}
+
+int fused_setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c)
+{
+    if (fs.fp32) setdt_t<float>(s, fs, g, d, c);
+    else setdt_t<double>(s, fs, g, d, c);
+    return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
+}
+
+template <typename T, bool MC, bool HLL>
+static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
+                    const StageArgs &sa, hipEvent_t ev0, hipEvent_t ev1)
+{
+    FArgs<T> a = make_args<T>(fs, g, d, c);
+    const bool last = sa.rk3 ? sa.stage == 3 : sa.stage == 2;
+    a.kind = sa.rk3 ? sa.stage : 3 + sa.stage;
+    a.last = last;
+    a.rk3 = sa.rk3;
+    a.want_dt = last;
+    a.counts = d.counts + 3 * (sa.stage - 1);
+    /* buffers: RK3 Un -> A -> B -> Un ; RK2 Un -> A -> Un */
+    const T *in = sa.stage == 1 ? (const T *)fs.Un : (sa.stage == 2 ? (const T *)fs.A : (const T *)fs.B);
+    T *out = last ? (T *)fs.Un : (sa.stage == 1 ? (T *)fs.A : (T *)fs.B);
+    a.Uin = in;
+    a.Uout = out;
+    a.nblocks = fs.tiles_j * fs.tiles_k * fs.nseg;
+    const int nb = (a.nblocks + 7) / 8 * 8;
+    const int ne = (g.NY - 2) * (g.NZ - 2);
+    hipLaunchKernelGGL((k_edge<T, MC, HLL>), dim3((ne + 255) / 256), dim3(256), 0, s, a, (T *)fs.edge);
+    if (ev0) hipEventRecord(ev0, s);
+    hipLaunchKernelGGL((k_fused<T, MC, HLL>), dim3(nb), dim3(NT), 0, s, a);
+    if (ev1) hipEventRecord(ev1, s);
+    const size_t nghost = 2 * (size_t)g.NY * g.NZ + (size_t)(g.NX - 2) * 2 * g.NZ + (size_t)(g.NX - 2) * (g.NY - 2) * 2;
+    hipLaunchKernelGGL(k_halo<T>, dim3(nblocks(nghost, 256, 4096)), dim3(256), 0, s, a);
+    if (last) fs.dt_valid = true;
+}
+
+int fused_stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
+                const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1)
+{
+    if (fs.fp32) {
+        if (a.mc) { if (a.hll) stage_t<float, true, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<float, true, false>(s, fs, g, d, c, a, ev0, ev1); }
+        else      { if (a.hll) stage_t<float, false, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<float, false, false>(s, fs, g, d, c, a, ev0, ev1); }
+    } else {
+        if (a.mc) { if (a.hll) stage_t<double, true, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<double, true, false>(s, fs, g, d, c, a, ev0, ev1); }
+        else      { if (a.hll) stage_t<double, false, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<double, false, false>(s, fs, g, d, c, a, ev0, ev1); }
+    }
+    return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
+}
+
+int fused_diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d)
+{
+    const size_t M = (size_t)(g.NX - 2) * (g.NY - 2) * (g.NZ - 2);
+    const int nb = (int)nblocks(M, 256, (unsigned)(d.nred / 4));
+    if (fs.fp32) hipLaunchKernelGGL(k_diag_t<float>, dim3(nb), dim3(256), 0, s, g, (const float *)fs.Un, d.N, d.red);
+    else hipLaunchKernelGGL(k_diag_t<double>, dim3(nb), dim3(256), 0, s, g, (const double *)fs.Un, d.N, d.red);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, d.red, nb);
+    return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
+}
+
+}  // namespace cyl

@@ -408,6 +408,74 @@ int cylmhd_model_disc(const cylmhd_params *p, double *V)
     return CYLMHD_OK;
 }
 
+/* Fill the ghost cells of U with the RK3 cell-boundary set (Y periodic, X
+ * axis, X right free, Z periodic: firststep.h:125-157) applied to the
+ * interior, and V there with UtoV (utov.h:24-68), so the initial state is
+ * consistent with what every later stage leaves in the ghosts. */
+static void fill_ghosts_rk3(const cylmhd_params *p, const cyl_grid *g, double *U, double *V)
+{
+    const int NX = g->NX, NY = g->NY, NZ = g->NZ, jy = p->jy;
+    const size_t N = (size_t)NX * NY * NZ;
+#define IX(i, j, k) ((((size_t)(i)) * NY + (size_t)(j)) * NZ + (size_t)(k))
+    for (int i = 0; i < NX; i++)
+        for (int j = 0; j < NY; j++)
+            for (int k = 0; k < NZ; k++) {
+                const int ghost = i == 0 || i == NX - 1 || j == 0 || j == NY - 1 || k == 0 || k == NZ - 1;
+                if (!ghost) continue;
+                int sk = k == 0 ? NZ - 2 : (k == NZ - 1 ? 1 : k), si, sj;
+                unsigned sign = 0;
+                if (i == 0) {
+                    si = 1;
+                    sj = j < jy / 2 + 1 ? j + jy / 2 : j - jy / 2;
+                    sign = (1u << 1) | (1u << 2) | (1u << 4) | (1u << 5);
+                } else {
+                    si = i == NX - 1 ? NX - 2 : i;
+                    sj = j == 0 ? NY - 2 : (j == NY - 1 ? 1 : j);
+                }
+                const size_t d = IX(i, j, k), s = IX(si, sj, sk);
+                for (int m = 0; m < 9; m++) U[m * N + d] = (sign >> m) & 1u ? -U[m * N + s] : U[m * N + s];
+                const double *u = U;
+                const double oneox = g->oneox[i];
+                const double oneorho = 1.0 / u[d];
+                const double b2 = u[4 * N + d] * u[4 * N + d] + u[5 * N + d] * u[5 * N + d] + u[6 * N + d] * u[6 * N + d];
+                const double r2v2 = u[1 * N + d] * u[1 * N + d] + u[2 * N + d] * u[2 * N + d] * oneox * oneox
+                                    + u[3 * N + d] * u[3 * N + d];
+                V[d] = u[d];
+                V[1 * N + d] = u[1 * N + d] * oneorho;
+                V[2 * N + d] = u[2 * N + d] * oneorho * oneox;
+                V[3 * N + d] = u[3 * N + d] * oneorho;
+                V[4 * N + d] = u[4 * N + d];
+                V[5 * N + d] = u[5 * N + d];
+                V[6 * N + d] = u[6 * N + d];
+                V[7 * N + d] = p->gm1 * (u[7 * N + d] - 0.5 * (r2v2 * oneorho + b2));
+                V[8 * N + d] = u[8 * N + d];
+            }
+#undef IX
+}
+
+int cylmhd_model_disc_state(const cylmhd_params *p, double *U, double *V)
+{
+    int rc;
+    if ((rc = cylmhd_model_disc(p, V))) return rc;
+    if ((rc = cylmhd_vtou_host(p, V, U))) return rc;
+    cyl_grid g;
+    if ((rc = cyl_grid_build(p, &g))) return rc;
+    fill_ghosts_rk3(p, &g, U, V);
+    cyl_grid_free(&g);
+    return CYLMHD_OK;
+}
+
+int cylmhd_fill_ghosts_host(const cylmhd_params *p, double *U, double *V)
+{
+    cyl_grid g;
+    int rc;
+    if (!U || !V) return CYLMHD_EINVAL;
+    if ((rc = cyl_grid_build(p, &g))) return rc;
+    fill_ghosts_rk3(p, &g, U, V);
+    cyl_grid_free(&g);
+    return CYLMHD_OK;
+}
+
 int cylmhd_model_galactic_disc3(const cylmhd_params *p, double *V)
 {
     /* model.h:164-292 (the z boundary calls on U at :312-313 are left to the

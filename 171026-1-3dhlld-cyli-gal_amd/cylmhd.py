@@ -91,6 +91,8 @@ def lib():
         "cylmhd_grid_host": ([P, _dp, _dp, _dp], C.c_int),
         "cylmhd_model_disc": ([P, _dp], C.c_int),
         "cylmhd_model_galactic_disc3": ([P, _dp], C.c_int),
+        "cylmhd_model_disc_state": ([P, _dp, _dp], C.c_int),
+        "cylmhd_fill_ghosts_host": ([P, _dp, _dp], C.c_int),
         "cylmhd_vtou_host": ([P, _dp, _dp], C.c_int),
         "cylmhd_create": ([P, C.POINTER(C.c_void_p)], C.c_int),
         "cylmhd_destroy": ([C.c_void_p], C.c_int),
@@ -158,6 +160,21 @@ def model_disc(p):
     V = np.zeros((9, NX, NY, NZ))
     check(lib().cylmhd_model_disc(C.byref(p), V), "model_disc")
     return V
+
+
+def disc_state(p):
+    """Consistent initial (U, V) of the synthetic disc (ghosts filled)."""
+    NX, NY, NZ = p.dims
+    U = np.zeros((9, NX, NY, NZ))
+    V = np.zeros((9, NX, NY, NZ))
+    check(lib().cylmhd_model_disc_state(C.byref(p), U, V), "model_disc_state")
+    return U, V
+
+
+def fill_ghosts(p, U, V):
+    """Ghosts of (U, V) in place by the RK3 cell-boundary set."""
+    check(lib().cylmhd_fill_ghosts_host(C.byref(p), U, V), "fill_ghosts")
+    return U, V
 
 
 def model_galactic_disc3(p):

@@ -100,6 +100,13 @@ int cylmhd_params_config(cylmhd_params *p, int config, int nr, int nphi, int nz)
 int cylmhd_grid_host(const cylmhd_params *p, double *x, double *y, double *z);
 /* Synthetic disc primitives V[9][NX][NY][NZ] (all cells incl. ghosts). */
 int cylmhd_model_disc(const cylmhd_params *p, double *V);
+/* Synthetic disc as a consistent (U, V) pair: interior from the disc,
+ * U = VtoU(V), ghosts by the RK3 cell-boundary set, ghost V = UtoV(U). */
+int cylmhd_model_disc_state(const cylmhd_params *p, double *U, double *V);
+/* Ghost cells of (U, V) from the interior by the RK3 cell-boundary set
+ * (Y periodic, X axis, X right free, Z periodic: firststep.h:125-157),
+ * ghost V = UtoV(ghost U).  Host arrays, in place. */
+int cylmhd_fill_ghosts_host(const cylmhd_params *p, double *U, double *V);
 /* GalacticDisc3 (model.h:149-327) primitives, without the U-side z BCs. */
 int cylmhd_model_galactic_disc3(const cylmhd_params *p, double *V);
 /* VtoU (vtou.h:20-62) on host arrays. */

@@ -119,20 +119,27 @@ int main(int argc, char **argv)
         return 2;
     }
     static double Vcur[9][ixmax][jymax][kzmax];
+    static double Uini[9][ixmax][jymax][kzmax], Ucur[9][ixmax][jymax][kzmax];
     static double grx[ixmax][kzmax], grz[ixmax][kzmax];
     rd(fi, V, sizeof V);
     rd(fi, Vcur, sizeof Vcur);
     rd(fi, grx, sizeof grx);
     rd(fi, grz, sizeof grz);
+    if (hdr[3] & 2) {            /* conserved states given as well */
+        rd(fi, Uini, sizeof Uini);
+        rd(fi, Ucur, sizeof Ucur);
+    }
     fclose(fi);
 
     CylindricalGrid();
     /* initial state -> Uinitial/Vinitial/rhofloor (saveinitial.h) */
-    VtoU();
+    if (hdr[3] & 2) memcpy(U, Uini, sizeof U);
+    else VtoU();
     SaveInitial();
     /* current state */
     memcpy(V, Vcur, sizeof V);
-    VtoU();
+    if (hdr[3] & 2) memcpy(U, Ucur, sizeof U);
+    else VtoU();
     if (hdr[3] & 1) {
         InitialGravity();
     } else {

@@ -189,8 +189,10 @@ def ref_harness(name):
     return path if os.path.isfile(path) else None
 
 
-def run_ref(name, V_init, V_cur, grx, grz, nsteps, rk, galactic=False, dumpdir=None):
-    """Run one compiled reference harness; returns (dict of arrays, stdout)."""
+def run_ref(name, V_init, V_cur, grx, grz, nsteps, rk, galactic=False, dumpdir=None, U_init=None, U_cur=None):
+    """Run one compiled reference harness; returns (dict of arrays, stdout).
+    With U_init/U_cur the harness takes the conserved states as given
+    instead of rebuilding them with VtoU."""
     exe = ref_harness(name)
     if exe is None:
         raise FileNotFoundError(name)
@@ -200,11 +202,15 @@ def run_ref(name, V_init, V_cur, grx, grz, nsteps, rk, galactic=False, dumpdir=N
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
-            np.array([NX, NY, NZ, 1 if galactic else 0], dtype=np.int64).tofile(f)
+            flags = (1 if galactic else 0) | (2 if U_init is not None else 0)
+            np.array([NX, NY, NZ, flags], dtype=np.int64).tofile(f)
             np.ascontiguousarray(V_init, dtype=np.float64).tofile(f)
             np.ascontiguousarray(V_cur, dtype=np.float64).tofile(f)
             np.ascontiguousarray(grx, dtype=np.float64).tofile(f)
             np.ascontiguousarray(grz, dtype=np.float64).tofile(f)
+            if U_init is not None:
+                np.ascontiguousarray(U_init, dtype=np.float64).tofile(f)
+                np.ascontiguousarray(U_cur, dtype=np.float64).tofile(f)
         args = [exe, fin, fout, str(nsteps), str(rk)]
         if dumpdir:
             args.append(dumpdir)
