@@ -35,6 +35,14 @@ struct Consts {          /* scalars of one stage, passed by value */
     double cool_lambda0, cool_tmin, cool_index;
 };
 
+/* A of the Raymond fit, 0.108e-37 (te0 T)^10.26 (cooling.h:50).  Kept out
+ * of line: it runs only inside the cooling band, and inlining the FP64 pow
+ * would reserve its registers across the whole stage update. */
+__device__ __noinline__ double raymond_A(double te0, double temperature)
+{
+    return 0.108e-37 * pow(te0 * temperature, 10.26);
+}
+
 /* ---- Cooling (cooling.h:22-60) --------------------------------------- */
 template <typename T>
 __device__ __forceinline__ T cooling(const Consts &c, T rho, T pr)
@@ -48,11 +56,15 @@ __device__ __forceinline__ T cooling(const Consts &c, T rho, T pr)
         const T tempmd = temperature - d, fabstempmd = fabs(tempmd);
         const T signa = tempma / fabstempma, signb = tempmb / fabstempmb;
         const T signc = tempmc / fabstempmc, signd = tempmd / fabstempmd;
-        const T A = T(0.108e-37) * T(pow(double(T(c.te0) * temperature), 10.26));
+        const T fa = T(1.0) - signa * signb;
         const T B = T(0.0), C = T(0.0);
+        /* outside the band a < T < b the factor of A is exactly 0 (and 0.5*
+         * (A*0 + B*.. + C*..) truncates to 0), so pow is only evaluated
+         * inside it; NaN factors (T equal to a break point) still flow */
+        if (fa == T(0.0)) return T(0.0);
+        const T A = T(raymond_A(double(T(c.te0)), double(temperature)));
         /* the reference returns through int */
-        return T((int)(T(0.5) * (A * (T(1.0) - signa * signb) + B * (T(1.0) - signb * signc)
-                                 + C * (T(1.0) - signc * signd))));
+        return T((int)(T(0.5) * (A * fa + B * (T(1.0) - signb * signc) + C * (T(1.0) - signc * signd))));
     }
     if (c.cooling == 2) {
         const T temperature = pr / rho;

@@ -40,7 +40,6 @@ namespace cyl {
 
 #define C3(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * g.NZ + (size_t)(k))
 
-constexpr int TJ = 15, TK = 16, NT = 256, NCELL = TJ * TK, NHALO = 4 * TK + 4 * TJ;
 
 /* ---- precision-specific UtoV and Check ---------------------------------- */
 template <typename T> struct Prec;
@@ -149,14 +148,24 @@ __device__ __forceinline__ int remap_block(int b, int nb)
     return (b % 8) * per + b / 8;
 }
 
+/* Tile shapes: FP32 15 x 16 cells / 256 threads, FP64 7 x 16 / 128 threads
+ * (the FP64 LDS ring is twice as large).  NT = number of phi faces. */
+template <typename T> struct Tile;
+template <> struct Tile<float>  { static constexpr int TJ = 15, TK = 16; };
+template <> struct Tile<double> { static constexpr int TJ = 7, TK = 16; };
+
 template <typename T, bool MC, bool HLL>
-__global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
+__global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs<T> a)
 {
+    constexpr int TJ = Tile<T>::TJ, TK = Tile<T>::TK;
+    constexpr int NT = (TJ + 1) * TK, NCELL = TJ * TK, NHALO = 4 * TK + 4 * TJ;
+    /* padded plane: (TJ+4) x (TK+4) cells (corners unused), 9 components,
+     * 3 ring slots; every stencil read is a per-thread base + an immediate */
+    constexpr int PK = TK + 4, PS = (TJ + 4) * PK, SLOT = 9 * PS;
     const DevGeom &g = a.g;
     const Consts &c = a.c;
-    __shared__ T ring[3][9][NCELL];
-    __shared__ T hal[9][NHALO];
-    __shared__ T fbuf[9][NT];
+    __shared__ T ring[3 * SLOT];
+    __shared__ T fbuf[9 * NT];
 
     const int t = threadIdx.x;
     const int b = remap_block(blockIdx.x, gridDim.x);
@@ -174,13 +183,19 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
     const int gj = j0 + cj, gk = k0 + ck;
     const bool valid = is_cell && gj <= g.NY - 2 && gk <= g.NZ - 2;
     const size_t N = a.N;
+    const int cbase = (cj + 2) * PK + ck + 2;           /* own cell in a padded plane */
 
+    auto ld9 = [&](int off, T *v) {
+#pragma unroll
+        for (int m = 0; m < 9; m++) v[m] = ring[off + m * PS];
+    };
     auto load_center = [&](int p) {
         if (is_cell && p >= 0 && p <= g.NX - 1) {
             T v[9];
             cell_V(a, C3(p, jmap(g, gj), kmap(g, gk)), p, v);
+            const int off = (p % 3) * SLOT + cbase;
 #pragma unroll
-            for (int m = 0; m < 9; m++) ring[p % 3][m][t] = v[m];
+            for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
         }
     };
     auto load_halo = [&](int p) {
@@ -192,24 +207,9 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
             else { const int u = t - 4 * TK - 2 * TJ; jj = u / 2; kk = TK + u % 2; }
             T v[9];
             cell_V(a, C3(p, jmap(g, j0 + jj), kmap(g, k0 + kk)), p, v);
+            const int off = (p % 3) * SLOT + (jj + 2) * PK + kk + 2;
 #pragma unroll
-            for (int m = 0; m < 9; m++) hal[m][t] = v[m];
-        }
-    };
-    /* V of plane in ring slot s at tile coords (jj, kk), halo included */
-    auto getv = [&](int s, int jj, int kk, T *v) {
-        int idx;
-        if (jj < 0) idx = (jj + 2) * TK + kk;
-        else if (jj >= TJ) idx = 2 * TK + (jj - TJ) * TK + kk;
-        else if (kk < 0) idx = 4 * TK + jj * 2 + (kk + 2);
-        else if (kk >= TK) idx = 4 * TK + 2 * TJ + jj * 2 + (kk - TK);
-        else idx = -1;
-        if (idx >= 0) {
-#pragma unroll
-            for (int m = 0; m < 9; m++) v[m] = hal[m][idx];
-        } else {
-#pragma unroll
-            for (int m = 0; m < 9; m++) v[m] = ring[s][m][jj * TK + kk];
+            for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
         }
     };
 
@@ -219,14 +219,13 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
     long long bix = LLONG_MAX;
     const size_t NE = (size_t)g.NY * g.NZ;
 
-    /* ---- prologue: states of cell ibeg and the flux at ibeg - 1/2 -------- */
+    /* ---- prologue: upper state of cell ibeg and the flux at ibeg - 1/2 --- */
     if (ibeg == 1) {
         load_center(0); load_center(1); load_center(2);
         __syncthreads();
         if (is_cell) {
             T vm[9], v0[9], vp[9];
-#pragma unroll
-            for (int m = 0; m < 9; m++) { vm[m] = ring[0][m][t]; v0[m] = ring[1][m][t]; vp[m] = ring[2][m][t]; }
+            ld9(0 * SLOT + cbase, vm); ld9(1 * SLOT + cbase, v0); ld9(2 * SLOT + cbase, vp);
             cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[1]), T(g.oneodx[0]), T(1.0), up_prev, (T *)nullptr);
             const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
 #pragma unroll
@@ -238,10 +237,8 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
         __syncthreads();
         if (is_cell) {
             T vm[9], v0[9], vp[9];
-#pragma unroll
-            for (int m = 0; m < 9; m++) {
-                vm[m] = ring[(ibeg - 2) % 3][m][t]; v0[m] = ring[(ibeg - 1) % 3][m][t]; vp[m] = ring[ibeg % 3][m][t];
-            }
+            ld9(((ibeg - 2) % 3) * SLOT + cbase, vm); ld9(((ibeg - 1) % 3) * SLOT + cbase, v0);
+            ld9((ibeg % 3) * SLOT + cbase, vp);
             cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[ibeg - 1]), T(g.oneodx[ibeg - 2]), T(1.0), upm, (T *)nullptr);
         }
         __syncthreads();
@@ -249,18 +246,11 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
         __syncthreads();
         if (is_cell) {
             T vm[9], v0[9], vp[9], lo[9];
-#pragma unroll
-            for (int m = 0; m < 9; m++) {
-                vm[m] = ring[(ibeg - 1) % 3][m][t]; v0[m] = ring[ibeg % 3][m][t]; vp[m] = ring[(ibeg + 1) % 3][m][t];
-            }
+            ld9(((ibeg - 1) % 3) * SLOT + cbase, vm); ld9((ibeg % 3) * SLOT + cbase, v0);
+            ld9(((ibeg + 1) % 3) * SLOT + cbase, vp);
             cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[ibeg]), T(g.oneodx[ibeg - 1]), T(1.0), up_prev, lo);
             riemann<0, HLL>(c, upm, lo, T(g.xm[ibeg - 1]), T(g.oneox[ibeg - 1]), fx_prev);
         }
-    }
-    if (ibeg == 1) {
-        /* ring holds 0,1,2; the loop below expects i, i+1 loaded: fine */
-    } else {
-        /* ring holds ibeg-1, ibeg, ibeg+1 */
     }
 
     for (int i = ibeg; i <= iend; i++) {
@@ -268,18 +258,17 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
         if (i + 2 <= g.NX - 1 && i < g.NX - 2) load_center(i + 2);
         load_halo(i);
         __syncthreads();
-        const int si = i % 3;
+        const int s0 = (i % 3) * SLOT, s1 = ((i + 1) % 3) * SLOT, s2 = ((i + 2) % 3) * SLOT;
         /* ---- x: flux at i+1/2 ------------------------------------------- */
-        T fxn[9];
         if (is_cell) {
+            T fxn[9];
             if (i == g.NX - 2) {
                 const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
 #pragma unroll
                 for (int m = 0; m < 9; m++) fxn[m] = a.edge[9 * NE + m * NE + e];   /* face NX-2 */
             } else {
                 T vm[9], v0[9], vp[9], lo[9], up[9];
-#pragma unroll
-                for (int m = 0; m < 9; m++) { vm[m] = ring[si][m][t]; v0[m] = ring[(i + 1) % 3][m][t]; vp[m] = ring[(i + 2) % 3][m][t]; }
+                ld9(s0 + cbase, vm); ld9(s1 + cbase, v0); ld9(s2 + cbase, vp);
                 cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[i + 1]), T(g.oneodx[i]), T(1.0), up, lo);
                 riemann<0, HLL>(c, up_prev, lo, T(g.xm[i]), T(g.oneox[i]), fxn);
 #pragma unroll
@@ -290,32 +279,35 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
 #pragma unroll
             for (int m = 0; m < 9; m++) { acc[m] = g2 * fxn[m] - g1 * fx_prev[m]; fx_prev[m] = fxn[m]; }
         }
-        /* ---- y: 16 x 16 faces ------------------------------------------- */
+        /* ---- y: (TJ+1) x TK faces --------------------------------------- */
         {
             const int fj = t / TK, fk = t % TK;          /* face between rows fj-1 and fj */
+            const int yb = s0 + fj * PK + fk + 2;        /* row fj-2 of column fk */
             T vm[9], v0[9], vp[9], vl[9], vr[9], fy[9];
             const T f1 = T(g.oneox[i]), f2 = T(g.oneody);
-            getv(si, fj - 2, fk, vm); getv(si, fj - 1, fk, v0); getv(si, fj, fk, vp);
+            ld9(yb, vm); ld9(yb + PK, v0); ld9(yb + 2 * PK, vp);
             cell_states<1, MC>(c, vm, v0, vp, f1, f1, f2, vl, (T *)nullptr);
-            getv(si, fj + 1, fk, vm);       /* reuse: rows fj-1, fj, fj+1 */
+            ld9(yb + 3 * PK, vm);                        /* rows fj-1, fj, fj+1 */
             cell_states<1, MC>(c, v0, vp, vm, f1, f1, f2, (T *)nullptr, vr);
             riemann<1, HLL>(c, vl, vr, T(g.x[i]), T(g.oneox[i]), fy);
 #pragma unroll
-            for (int m = 0; m < 9; m++) fbuf[m][t] = fy[m];
+            for (int m = 0; m < 9; m++) fbuf[m * NT + t] = fy[m];
         }
         __syncthreads();
         if (is_cell) {
             const size_t ik = (size_t)i * g.NZ + kmap(g, gk);
             const T g3 = T(g.g3[ik]), g4 = g3;
 #pragma unroll
-            for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g4 * fbuf[m][(cj + 1) * TK + ck] - g3 * fbuf[m][cj * TK + ck]);
+            for (int m = 0; m < 9; m++)
+                acc[m] = acc[m] + (g4 * fbuf[m * NT + (cj + 1) * TK + ck] - g3 * fbuf[m * NT + cj * TK + ck]);
         }
         __syncthreads();
-        /* ---- z: 15 x 17 faces ------------------------------------------- */
+        /* ---- z: TJ x (TK+1) faces --------------------------------------- */
         if (t < TJ * (TK + 1)) {
             const int fj = t / (TK + 1), fk = t % (TK + 1);   /* face between cols fk-1 and fk */
             const int kf = k0 + fk - 1;                       /* global face index */
             const int jgl = jmap(g, j0 + fj);
+            const int zb = s0 + (fj + 2) * PK + fk;           /* col fk-2 of row fj */
             T vm[9], v0[9], vp[9], vl[9], vr[9], fz[9];
             if (kf == 0) {
                 if (MC) {
@@ -323,14 +315,14 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
 #pragma unroll
                     for (int m = 0; m < 9; m++) vl[m] = T(a.Vinit_z[(size_t)m * 2 * NP + (size_t)i * g.NY + jgl]);
                 } else {
-                    /* lower state of cell 2 (col fk+1), vz negated */
-                    getv(si, fj, fk, vm); getv(si, fj, fk + 1, v0); getv(si, fj, fk + 2, vp);
+                    /* lower state of cell 2 (cols fk..fk+2), vz negated */
+                    ld9(zb + 2, vm); ld9(zb + 3, v0); ld9(zb + 4, vp);
                     cell_states<2, MC>(c, vm, v0, vp, T(g.oneodz[2]), T(g.oneodz[1]), T(1.0), (T *)nullptr, vl);
                     vl[3] = -vl[3];
                 }
             } else {
-                getv(si, fj, fk - 2, vm); getv(si, fj, fk - 1, v0); getv(si, fj, fk, vp);
                 const int kc = kmap(g, kf);
+                ld9(zb, vm); ld9(zb + 1, v0); ld9(zb + 2, vp);
                 cell_states<2, MC>(c, vm, v0, vp, T(g.oneodz[kc]), T(g.oneodz[kc > 0 ? kc - 1 : 0]), T(1.0), vl, (T *)nullptr);
             }
             if (kf == g.NZ - 2) {
@@ -343,13 +335,13 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
                     for (int m = 0; m < 9; m++) vr[m] = vl[m];
                 }
             } else {
-                getv(si, fj, fk - 1, vm); getv(si, fj, fk, v0); getv(si, fj, fk + 1, vp);
                 const int kc = kmap(g, kf + 1);
+                ld9(zb + 1, vm); ld9(zb + 2, v0); ld9(zb + 3, vp);
                 cell_states<2, MC>(c, vm, v0, vp, T(g.oneodz[kc]), T(g.oneodz[kc > 0 ? kc - 1 : 0]), T(1.0), (T *)nullptr, vr);
             }
             riemann<2, HLL>(c, vl, vr, T(g.x[i]), T(g.oneox[i]), fz);
 #pragma unroll
-            for (int m = 0; m < 9; m++) fbuf[m][t] = fz[m];
+            for (int m = 0; m < 9; m++) fbuf[m * NT + t] = fz[m];
         }
         __syncthreads();
         /* ---- divergence, sources, stage update, absorb, Check, store ----- */
@@ -358,10 +350,9 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
             const T g5 = T(g.g5[ik]), g6 = g5;
 #pragma unroll
             for (int m = 0; m < 9; m++)
-                acc[m] = acc[m] + (g6 * fbuf[m][cj * (TK + 1) + ck + 1] - g5 * fbuf[m][cj * (TK + 1) + ck]);
+                acc[m] = acc[m] + (g6 * fbuf[m * NT + cj * (TK + 1) + ck + 1] - g5 * fbuf[m * NT + cj * (TK + 1) + ck]);
             T v[9], S[9], u[9], un[9];
-#pragma unroll
-            for (int m = 0; m < 9; m++) v[m] = ring[si][m][t];
+            ld9(s0 + cbase, v);
             double gx, gy, gz;
             const bool spiral = gravity_cell(g, i, gj, gk, gx, gy, gz);
             sources(c, v, T(g.x[i]), T(g.oneox[i]), T(gx), T(gy), T(gz), spiral, S);
@@ -430,7 +421,7 @@ __global__ __launch_bounds__(NT) void k_fused(FArgs<T> a)
     }
     if (a.want_dt) {
         __syncthreads();
-        double *sv = (double *)&fbuf[0][0];
+        double *sv = (double *)&fbuf[0];
         long long *sx = (long long *)(sv + NT);
         sv[t] = best; sx[t] = bix;
         __syncthreads();
@@ -683,6 +674,8 @@ int fused_alloc(FusedState &fs, const DevGeom &g, const DevState &d, bool fp32, 
     void *p;
     if ((rc = alloc(9 * N * sizeof(double), &p))) return rc;
     fs.u64_scratch = (double *)p;
+    const int TJ = fp32 ? Tile<float>::TJ : Tile<double>::TJ;
+    const int TK = fp32 ? Tile<float>::TK : Tile<double>::TK;
     fs.tiles_j = (g.NY - 2 + TJ - 1) / TJ;
     fs.tiles_k = (g.NZ - 2 + TK - 1) / TK;
     const int cols = fs.tiles_j * fs.tiles_k;
@@ -782,7 +775,7 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
     const int ne = (g.NY - 2) * (g.NZ - 2);
     hipLaunchKernelGGL((k_edge<T, MC, HLL>), dim3((ne + 255) / 256), dim3(256), 0, s, a, (T *)fs.edge);
     if (ev0) hipEventRecord(ev0, s);
-    hipLaunchKernelGGL((k_fused<T, MC, HLL>), dim3(nb), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((k_fused<T, MC, HLL>), dim3(nb), dim3((Tile<T>::TJ + 1) * Tile<T>::TK), 0, s, a);
     if (ev1) hipEventRecord(ev1, s);
     const size_t nghost = 2 * (size_t)g.NY * g.NZ + (size_t)(g.NX - 2) * 2 * g.NZ + (size_t)(g.NX - 2) * (g.NY - 2) * 2;
     hipLaunchKernelGGL(k_halo<T>, dim3(nblocks(nghost, 256, 4096)), dim3(256), 0, s, a);

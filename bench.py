@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""bench.py -- cell-updates/s of the RK3 step on one MI355X.
+
+Workload (the metric's own configuration): the 512x512x256 disc -- FP32 state
+with FP64 pressure recovery, conserved sums and CFL reduction, MC + HLLD,
+TVD-RK3, spiral gravity m=2 at 5 %, beta log-uniform in [0.1, 100] per radial
+annulus.  A step is the reference's main-loop body: Setdt (with its
+device->host dt), SpiralGravity, TVDRK.  The same disc is also timed on the
+two FP64 paths (fused, and the plain pass-by-pass drop-in) for the FP32 vs
+FP64 comparison.
+
+value     = interior cells x K / wall seconds of K steps (state resident in
+            HBM; host<->device traffic is only the scalars of Setdt).
+roofline  = the fused stage kernel: algorithmic HBM bytes per launch
+            (288 B per cell-update / 3 launches, FP32 state) / its average
+            duration from HIP events on the library's stream.
+cpu_baseline = the CPU restatement of the reference (oracle, OpenMP on all
+            host cores) or the reference itself (oracle/_ref bench build),
+            on a bounded sample of the same disc.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "171026-1-3dhlld-cyli-gal_amd")
+sys.path[:0] = [PKG, os.path.join(ROOT, "tests")]
+
+HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
+METRIC = "cell-updates/s per RK3 step (FP32 vs FP64 path) + % HBM roofline, 1 MI355X"
+BYTES_PER_CELL_UPDATE = {"fused32": 288.0, "fused64": 576.0, "plain64": 576.0}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_path(cm, np, path, grid, steps, warmup):
+    p = cm.params_config(3, *grid)
+    p.path = path
+    t0 = time.perf_counter()
+    U, V = cm.disc_state(p)
+    log(f"[bench] {cm.PATH_NAMES[path]}: host init {time.perf_counter() - t0:.1f}s")
+    ctx = cm.Context(p)
+    ctx.set_initial(U, V)
+    del U, V
+    t = 0.0
+    for _ in range(warmup):
+        dt, ch, _ = ctx.setdt()
+        ctx.gravity(t)
+        ctx.step(dt, ch)
+        t += dt
+    ctx.sync()
+    ctx.kernel_timing(0, reset=True)          # start timing the stage kernels
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dt, ch, _ = ctx.setdt()
+        ctx.gravity(t)
+        ctx.step(dt, ch)
+        t += dt
+    ctx.sync()
+    el = time.perf_counter() - t0
+    kms, nlaunch = ctx.kernel_timing(0, reset=True)
+    diag = ctx.diagnostics()
+    counts = ctx.floor_counts()
+    ctx.close()
+    ncell = p.ix * p.jy * p.kz
+    name = cm.PATH_NAMES[path]
+    res = dict(path=name, value=ncell * steps / el, ms_per_step=1e3 * el / steps, steps=steps,
+               kernel_ms=kms, kernel_launches=nlaunch, interior_cells=ncell, t=t,
+               mass=diag["mass"], energy=diag["energy"], divb_l1=diag["divb_l1"],
+               pressure_floor_hits_last_step=int(counts[:, 1].sum()))
+    log(f"[bench] {name}: {res['value']:.4g} cell-updates/s, {res['ms_per_step']:.2f} ms/step, "
+        f"stage kernel {kms:.3f} ms x {nlaunch}")
+    return res
+
+
+def cpu_baseline(cm, np, budget_s=15.0):
+    """CPU restatement of the reference on a bounded sample: the same disc at
+    128x128x64, as many whole steps as fit the time budget (at least 1)."""
+    from oracle_py import Oracle
+    grid = (128, 128, 64)
+    p = cm.params_config(3, *grid)
+    U, V = cm.disc_state(p)
+    o = Oracle(p)
+    o.set_initial(U, V)
+    threads = o.L.orc_threads()
+    t, el = o.run(1)                          # first step also warms the caches
+    n = max(1, min(20, int(budget_s / max(el, 1e-3))))
+    t, el = o.run(n, t)
+    ncell = p.ix * p.jy * p.kz
+    o.close()
+    return dict(value=ncell * n / el, unit="cell-updates/s", cores=threads, kind="port",
+                sample=f"same disc at {grid[0]}x{grid[1]}x{grid[2]} (FP64, MC+HLLD, RK3), {n} steps after 1 "
+                       f"warm-up step, oracle/liboracle.so with OpenMP on {threads} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, nargs=3, default=[512, 512, 256])
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison paths")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); the path would shard as "
+              f"phi-slabs with one-cell halo exchanges (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import cylmhd as cm
+
+    grid = tuple(a.grid)
+    main_res = run_path(cm, np, cm.PATH_FUSED32, grid, a.steps, a.warmup)
+    fp64 = {}
+    if not a.no_fp64:
+        k64 = max(2, a.steps // 2)
+        for path in (cm.PATH_FUSED64, cm.PATH_PLAIN64):
+            fp64[cm.PATH_NAMES[path]] = run_path(cm, np, path, grid, k64, 1)
+    ncell = main_res["interior_cells"]
+    bytes_launch = BYTES_PER_CELL_UPDATE["fused32"] * ncell / 3.0
+    achieved = bytes_launch / (main_res["kernel_ms"] * 1e-3) if main_res["kernel_ms"] > 0 else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_fused32_{grid[0]}x{grid[1]}x{grid[2]}.json")
+    if os.path.isfile(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roofline = dict(bound="hbm", achieved=achieved / 1e9 if achieved else None, peak=HBM_PEAK / 1e9, unit="GB/s",
+                    frac=achieved / HBM_PEAK if achieved else None, traffic=traffic,
+                    kernel="k_fused<float,MC,HLLD>", algorithmic_bytes_per_launch=bytes_launch,
+                    kernel_ms=main_res["kernel_ms"])
+    out = {
+        "metric": METRIC,
+        "value": main_res["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": main_res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"disc {grid[0]}x{grid[1]}x{grid[2]}: FP32 state + FP64 recovery, MC+HLLD, RK3, "
+                               "spiral m=2 5%, beta log-uniform [0.1,100] per annulus",
+                   "grid": list(grid), "interior_cells": ncell, "path": "fused32"},
+        "roofline": roofline,
+    }
+    if fp64:
+        out["fp64"] = {k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms", "steps")} for k, v in fp64.items()}
+        out["fp32_over_fp64"] = {k: main_res["value"] / v["value"] for k, v in fp64.items()}
+    if not a.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(cm, np)
+        except Exception as e:                     # reported, never fatal to the GPU number
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
