@@ -109,6 +109,15 @@ int validate(const cylmhd_params *p)
     }
     if ((int)(p->rin / p->dx0) + 1 > p->ix + 2) { snprintf(g_errmsg, sizeof g_errmsg, "rin too large"); return CYLMHD_EINVAL; }
     if (p->path < CYLMHD_PATH_PLAIN64 || p->path > CYLMHD_PATH_FUSED32) return CYLMHD_EINVAL;
+    if (p->limiter < CYLMHD_LIMITER_MINMOD || p->limiter > CYLMHD_LIMITER_SUPERBEE || p->solver < CYLMHD_SOLVER_HLLD ||
+        p->solver > CYLMHD_SOLVER_HLL) {
+        snprintf(g_errmsg, sizeof g_errmsg, "unknown limiter %d or solver %d", p->limiter, p->solver);
+        return CYLMHD_EINVAL;
+    }
+    if (p->path != CYLMHD_PATH_PLAIN64 && p->limiter > CYLMHD_LIMITER_MC) {
+        snprintf(g_errmsg, sizeof g_errmsg, "the vanLeer and superbee limiters run on the plain FP64 path only");
+        return CYLMHD_EUNSUP;
+    }
     return CYLMHD_OK;
 }
 
@@ -186,6 +195,7 @@ int alloc_state(cylmhd_ctx *c)
     d.NF = (size_t)(NX - 1) * (NY - 1) * (NZ - 1);
     int rc;
     if ((rc = dalloc(c, &d.Uinit, 9 * d.N)) || (rc = dalloc(c, &d.Vinit_z, 9 * 2 * (size_t)NX * NY)) ||
+        (rc = dalloc(c, &d.Vinit_y, 9 * 2 * (size_t)NX * NZ)) ||
         (rc = dalloc(c, &d.counts, 9)))
         return rc;
     d.nred = 8192;
@@ -298,6 +308,15 @@ int cylmhd_set_initial(cylmhd_ctx *c, const double *U, const double *V)
             vz[(size_t)m * 2 * NP + NP + ij] = V[(size_t)m * N + ij * NZ + (NZ - 1)];
         }
     HIPCHK(hipMemcpy(c->ds.Vinit_z, vz.data(), vz.size() * sizeof(double), hipMemcpyHostToDevice));
+    const size_t NQ = (size_t)NX * NZ;
+    std::vector<double> vy(9 * 2 * NQ);
+    for (int m = 0; m < 9; m++)
+        for (int i = 0; i < NX; i++)
+            for (int k = 0; k < NZ; k++) {
+                vy[(size_t)m * 2 * NQ + (size_t)i * NZ + k] = V[(size_t)m * N + ((size_t)i * NY + 0) * NZ + k];
+                vy[(size_t)m * 2 * NQ + NQ + (size_t)i * NZ + k] = V[(size_t)m * N + ((size_t)i * NY + NY - 1) * NZ + k];
+            }
+    HIPCHK(hipMemcpy(c->ds.Vinit_y, vy.data(), vy.size() * sizeof(double), hipMemcpyHostToDevice));
     if (c->p.path != CYLMHD_PATH_PLAIN64) {
         int rc = fused_set_rhofloor(c->stream, c->fs, c->ds, c->p.rhocr);
         if (rc) return rc;
@@ -374,7 +393,7 @@ int cylmhd_step(cylmhd_ctx *c, double dt, double ch)
             e1 = c->ev[c->ev_used][1];
             c->ev_used++;
         }
-        StageArgs a{s, rk3, c->p.limiter == CYLMHD_LIMITER_MC, c->p.solver == CYLMHD_SOLVER_HLL};
+        StageArgs a{s, rk3, c->p.limiter == CYLMHD_LIMITER_MC, c->p.solver == CYLMHD_SOLVER_HLL, c->p.limiter};
         if (c->p.path == CYLMHD_PATH_PLAIN64) {
             plain_stage(c->stream, c->dg, c->ds, k, a, e0, e1);
         } else {

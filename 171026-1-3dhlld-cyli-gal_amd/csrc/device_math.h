@@ -79,16 +79,18 @@ __device__ __forceinline__ T cooling(const Consts &c, T rho, T pr)
 /* ---- limiter interface states ------------------------------------------
  * upper = state at the cell's +side face (Vl of that face),
  * lower = state at the cell's -side face (Vr of that face).
- * Minmod: limiterx.h:171-217 / limitery.h:46-94 / limiterz.h:146-190
- * (MUSCL-Hancock factor 0.5*(1 -/+ lambda dt/dx)); the y factor is
- * ((lambda*dt)*oneox)*oneody, x and z pass f2 = 1 (exact).
- * MC: limiterx.h:335-385 / limitery.h:197-247 / limiterz.h:294-358.        */
-template <int DIR, bool MC, typename T>
+ * LIM 0 Minmod: limiterx.h:171-217 / limitery.h:46-94 / limiterz.h:146-190
+ *   (MUSCL-Hancock factor 0.5*(1 -/+ lambda dt/dx)); the y factor is
+ *   ((lambda*dt)*oneox)*oneody, x and z pass f2 = 1 (exact).
+ * LIM 1 MC: limiterx.h:335-385 / limitery.h:197-247 / limiterz.h:294-358.
+ * LIM 2 vanLeer: limiterx.h:266-326 / limitery.h:129-188 / limiterz.h:232-285.
+ * LIM 3 superbee: limiterx.h:394-449 / limitery.h:256-311 / limiterz.h:367-432. */
+template <int DIR, int LIM, typename T>
 __device__ __forceinline__ void cell_states(const Consts &c, const T *vm, const T *v0, const T *vp,
                                             T f1hi, T f1lo, T f2, T *upper, T *lower)
 {
     T lambmax = T(0), lambmin = T(0);
-    if (!MC) {
+    if (LIM == 0) {
         const int nb = 4 + DIR, nv = 1 + DIR;
         const T b2 = (v0[4] * v0[4] + v0[5] * v0[5] + v0[6] * v0[6]);
         const T gmpr = T(c.gm) * v0[7];
@@ -103,13 +105,24 @@ __device__ __forceinline__ void cell_states(const Consts &c, const T *vm, const 
         const T a = v0[m] - vm[m];
         const T b = vp[m] - v0[m];
         T grad;
-        if (MC) {
+        if (LIM == 1) {
             const T cc = T(0.25) * (vp[m] - vm[m]);
             if (a * b <= T(0.0)) grad = T(0.0);
             else if (a > T(0.0)) grad = rmin(a, rmin(b, cc));
             else grad = rmax(a, rmax(b, cc));
             if (upper) upper[m] = v0[m] + grad;
             if (lower) lower[m] = v0[m] - grad;
+        } else if (LIM == 2) {
+            if (a * b <= T(0.0)) grad = T(0.0);
+            else grad = a * b / (a + b);
+            if (upper) upper[m] = v0[m] + grad;
+            if (lower) lower[m] = v0[m] - grad;
+        } else if (LIM == 3) {
+            if (a * b <= T(0.0)) grad = T(0.0);
+            else if (fabs(a) >= fabs(b)) grad = a > T(0.0) ? rmin(a, T(2.0) * b) : rmax(a, T(2.0) * b);
+            else grad = a > T(0.0) ? rmin(T(2.0) * a, b) : rmax(T(2.0) * a, b);
+            if (upper) upper[m] = v0[m] + T(0.5) * grad;
+            if (lower) lower[m] = v0[m] - T(0.5) * grad;
         } else {
             if (a * b <= T(0.0)) grad = T(0.0);
             else if (a > T(0.0)) grad = rmin(a, b);

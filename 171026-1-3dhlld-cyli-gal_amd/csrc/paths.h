@@ -14,8 +14,9 @@ struct DevState {
     /* plain FP64 path (reference data flow: U, U1, V, F, G, H) */
     double *U, *U1, *V, *F, *G, *H;
     /* initial state: Uinit for the absorbing layers and rho floor,
-     * Vinit planes k = 0 and k = NZ-1 for the MC z boundary */
-    double *Uinit, *Vinit_z;       /* [9][N], [9][2][NX][NY] */
+     * Vinit planes k = 0 and k = NZ-1 for the MC z boundary and
+     * j = 0 and j = NY-1 for the vanLeer y boundary */
+    double *Uinit, *Vinit_z, *Vinit_y;  /* [9][N], [9][2][NX][NY], [9][2][NX][NZ] */
     int *counts;                   /* [9] floor counters of the last step */
     double *red;                   /* reduction scratch */
     int nred;
@@ -28,6 +29,7 @@ struct DevState {
 struct StageArgs {
     int stage;          /* 1..3 (RK3) or 1..2 (RK2) */
     bool rk3, mc, hll;
+    int limiter;        /* CYLMHD_LIMITER_* */
 };
 
 /* plain FP64 path (plain.hip) */

@@ -3,7 +3,7 @@
  * reference routine, every pass a gfx950 kernel.
  *
  *   k_glm          GLMSourceSolver   (glmsourcesolver.h:19-47)
- *   k_flux<DIR>    Minmod/MC limiter + interface BCs + HLLD/HLL + GLM flux,
+ *   k_flux<DIR>    Minmod/MC/vanLeer/superbee limiter + interface BCs + HLLD/HLL + GLM flux,
  *                  one thread per face (limiterx.h / hlldx.h and y/z twins)
  *   k_update<KIND> sources (source.h:26-134) + stage update
  *                  (firststep.h:97-118 ...)
@@ -38,7 +38,7 @@ __device__ __forceinline__ void load9(const double *__restrict__ A, size_t N, si
 }
 
 /* upper (+face) or lower (-face) interface state of cell (ci,cj,ck) */
-template <int DIR, bool MC, bool UPPER>
+template <int DIR, int LIM, bool UPPER>
 __device__ __forceinline__ void face_state(const DevGeom &g, const Consts &c, const double *__restrict__ V,
                                            size_t N, int ci, int cj, int ck, double *out)
 {
@@ -52,50 +52,82 @@ __device__ __forceinline__ void face_state(const DevGeom &g, const Consts &c, co
     if (DIR == 0) f1 = UPPER ? g.oneodx[ci] : g.oneodx[ci - 1];
     else if (DIR == 1) { f1 = g.oneox[ci]; f2 = g.oneody; }
     else f1 = UPPER ? g.oneodz[ck] : g.oneodz[ck - 1];
-    if (UPPER) cell_states<DIR, MC>(c, vm, v0, vp, f1, f1, f2, out, (double *)nullptr);
-    else cell_states<DIR, MC>(c, vm, v0, vp, f1, f1, f2, (double *)nullptr, out);
+    if (UPPER) cell_states<DIR, LIM>(c, vm, v0, vp, f1, f1, f2, out, (double *)nullptr);
+    else cell_states<DIR, LIM>(c, vm, v0, vp, f1, f1, f2, (double *)nullptr, out);
 }
 
-template <int DIR, bool MC, bool HLL>
+/* One thread per face: left/right states from the limiter of the two cells
+ * that share the face, with the interface-state boundary the limiter applies
+ * on the first and last face (xvlrboundary.h, yvlrboundary.h, zvlrboundary.h
+ * as called from limiter{x,y,z}.h), then the Riemann flux. */
+template <int DIR, int LIM, bool HLL>
 __global__ __launch_bounds__(256) void k_flux(DevGeom g, Consts c, const double *__restrict__ V,
-                                              const double *__restrict__ Vinit_z, double *__restrict__ Fo,
-                                              size_t N, size_t NF)
+                                              const double *__restrict__ Vinit_z, const double *__restrict__ Vinit_y,
+                                              double *__restrict__ Fo, size_t N, size_t NF)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x + (DIR == 2 ? 0 : 1);
     const int j = blockIdx.y * blockDim.y + threadIdx.y + (DIR == 1 ? 0 : 1);
     const int i = blockIdx.z + (DIR == 0 ? 0 : 1);
-    const int kmax = DIR == 2 ? g.NZ - 2 : g.NZ - 2;   /* inclusive upper bounds */
+    const int kmax = g.NZ - 2;   /* inclusive upper bounds */
     const int jmax = g.NY - 2;
     if (k > kmax || j > jmax) return;
     double vl[9], vr[9], fx[9];
     if (DIR == 0) {
         const int last = g.NX - 2;
-        face_state<0, MC, true>(g, c, V, N, i == 0 ? last : i, j, k, vl);       /* XVlrPeriodic */
-        face_state<0, MC, false>(g, c, V, N, i == last ? 1 : i + 1, j, k, vr);
+        if (LIM == 2) {
+            /* XVlrFreeBoundary: Vl[0] = Vr[0], Vr[NX-2] = Vl[NX-2] */
+            if (i == 0) face_state<0, LIM, false>(g, c, V, N, 1, j, k, vl);
+            else face_state<0, LIM, true>(g, c, V, N, i, j, k, vl);
+            if (i == last) face_state<0, LIM, true>(g, c, V, N, last, j, k, vr);
+            else face_state<0, LIM, false>(g, c, V, N, i + 1, j, k, vr);
+        } else {
+            face_state<0, LIM, true>(g, c, V, N, i == 0 ? last : i, j, k, vl);       /* XVlrPeriodic */
+            face_state<0, LIM, false>(g, c, V, N, i == last ? 1 : i + 1, j, k, vr);
+        }
     } else if (DIR == 1) {
         const int last = g.NY - 2;
-        face_state<1, MC, true>(g, c, V, N, i, j == 0 ? last : j, k, vl);       /* YVlrPeriodic */
-        face_state<1, MC, false>(g, c, V, N, i, j == last ? 1 : j + 1, k, vr);
+        if (LIM == 2 && j == 0) {
+            /* YVlrInitializeBoundary after the periodic copy: Vinitial at j = 0 */
+            for (int m = 0; m < 9; m++) vl[m] = Vinit_y[(size_t)m * 2 * g.NX * g.NZ + (size_t)i * g.NZ + k];
+        } else {
+            face_state<1, LIM, true>(g, c, V, N, i, j == 0 ? last : j, k, vl);       /* YVlrPeriodic */
+        }
+        if (LIM == 2 && j == last) {
+            for (int m = 0; m < 9; m++)
+                vr[m] = Vinit_y[(size_t)m * 2 * g.NX * g.NZ + (size_t)g.NX * g.NZ + (size_t)i * g.NZ + k];
+        } else {
+            face_state<1, LIM, false>(g, c, V, N, i, j == last ? 1 : j + 1, k, vr);
+        }
     } else {
         const int last = g.NZ - 2;
         const size_t NP = (size_t)g.NX * g.NY;
-        if (MC) {
+        if (LIM == 1) {
             /* ZVlrFixedBoundary: Vinitial at k = 0 and k = NZ-1 */
             if (k == 0) { for (int m = 0; m < 9; m++) vl[m] = Vinit_z[(size_t)m * 2 * NP + (size_t)i * g.NY + j]; }
-            else face_state<2, MC, true>(g, c, V, N, i, j, k, vl);
+            else face_state<2, LIM, true>(g, c, V, N, i, j, k, vl);
             if (k == last) { for (int m = 0; m < 9; m++) vr[m] = Vinit_z[(size_t)m * 2 * NP + NP + (size_t)i * g.NY + j]; }
-            else face_state<2, MC, false>(g, c, V, N, i, j, k + 1, vr);
+            else face_state<2, LIM, false>(g, c, V, N, i, j, k + 1, vr);
+        } else if (LIM == 2) {
+            /* ZVlrFreeBoundary: Vl[0] = Vr[0], Vr[NZ-2] = Vl[NZ-2] */
+            if (k == 0) face_state<2, LIM, false>(g, c, V, N, i, j, 1, vl);
+            else face_state<2, LIM, true>(g, c, V, N, i, j, k, vl);
+            if (k == last) face_state<2, LIM, true>(g, c, V, N, i, j, last, vr);
+            else face_state<2, LIM, false>(g, c, V, N, i, j, k + 1, vr);
+        } else if (LIM == 3) {
+            /* ZVlrPeriodicBoundary: Vl[0] = Vl[NZ-2], Vr[NZ-2] = Vr[0] */
+            face_state<2, LIM, true>(g, c, V, N, i, j, k == 0 ? last : k, vl);
+            face_state<2, LIM, false>(g, c, V, N, i, j, k == last ? 1 : k + 1, vr);
         } else {
             /* ZLeftVlVzAntisymBzsym: Vl[0] = Vr[1] (the lower state of cell 2),
              * vz negated; ZRightVrFree: Vr[NZ-2] = Vl[NZ-2] */
             if (k == 0) {
-                face_state<2, MC, false>(g, c, V, N, i, j, 2, vl);
+                face_state<2, LIM, false>(g, c, V, N, i, j, 2, vl);
                 vl[3] = -vl[3];
             } else {
-                face_state<2, MC, true>(g, c, V, N, i, j, k, vl);
+                face_state<2, LIM, true>(g, c, V, N, i, j, k, vl);
             }
             if (k == last) { for (int m = 0; m < 9; m++) vr[m] = vl[m]; }
-            else face_state<2, MC, false>(g, c, V, N, i, j, k + 1, vr);
+            else face_state<2, LIM, false>(g, c, V, N, i, j, k + 1, vr);
         }
     }
     const double geo = DIR == 0 ? g.xm[i] : g.x[i];
@@ -247,22 +279,22 @@ static unsigned grid1d(size_t n, unsigned bs, unsigned cap = 65536u)
     return (unsigned)b;
 }
 
-template <int DIR, bool MC, bool HLL>
+template <int DIR, int LIM, bool HLL>
 static void launch_flux(hipStream_t s, const DevGeom &g, const Consts &c, const DevState &d, double *Fo)
 {
     const int nk = DIR == 2 ? g.NZ - 1 : g.NZ - 2;
     const int nj = DIR == 1 ? g.NY - 1 : g.NY - 2;
     const int ni = DIR == 0 ? g.NX - 1 : g.NX - 2;
     dim3 bs(64, 4, 1), gs((nk + 63) / 64, (nj + 3) / 4, ni);
-    hipLaunchKernelGGL((k_flux<DIR, MC, HLL>), gs, bs, 0, s, g, c, d.V, d.Vinit_z, Fo, d.N, d.NF);
+    hipLaunchKernelGGL((k_flux<DIR, LIM, HLL>), gs, bs, 0, s, g, c, d.V, d.Vinit_z, d.Vinit_y, Fo, d.N, d.NF);
 }
 
-template <bool MC, bool HLL>
+template <int LIM, bool HLL>
 static void launch_fluxes(hipStream_t s, const DevGeom &g, const Consts &c, const DevState &d)
 {
-    launch_flux<0, MC, HLL>(s, g, c, d, d.F);
-    launch_flux<1, MC, HLL>(s, g, c, d, d.G);
-    launch_flux<2, MC, HLL>(s, g, c, d, d.H);
+    launch_flux<0, LIM, HLL>(s, g, c, d, d.F);
+    launch_flux<1, LIM, HLL>(s, g, c, d, d.G);
+    launch_flux<2, LIM, HLL>(s, g, c, d, d.H);
 }
 
 void plain_stage(hipStream_t s, const DevGeom &g, const DevState &d, const Consts &c, const StageArgs &a,
@@ -274,8 +306,16 @@ void plain_stage(hipStream_t s, const DevGeom &g, const DevState &d, const Const
     double *Uout = last ? d.U : d.U1;
     hipLaunchKernelGGL(k_glm, dim3(grid1d(N, 256, 8192)), dim3(256), 0, s, Uin + 8 * N, d.V + 8 * N, N, c.psicoef);
     if (ev0) hipEventRecord(ev0, s);
-    if (a.mc) { if (a.hll) launch_fluxes<true, true>(s, g, c, d); else launch_fluxes<true, false>(s, g, c, d); }
-    else      { if (a.hll) launch_fluxes<false, true>(s, g, c, d); else launch_fluxes<false, false>(s, g, c, d); }
+    switch (a.limiter * 2 + (a.hll ? 1 : 0)) {
+    case 0: launch_fluxes<0, false>(s, g, c, d); break;
+    case 1: launch_fluxes<0, true>(s, g, c, d); break;
+    case 2: launch_fluxes<1, false>(s, g, c, d); break;
+    case 3: launch_fluxes<1, true>(s, g, c, d); break;
+    case 4: launch_fluxes<2, false>(s, g, c, d); break;
+    case 5: launch_fluxes<2, true>(s, g, c, d); break;
+    case 6: launch_fluxes<3, false>(s, g, c, d); break;
+    default: launch_fluxes<3, true>(s, g, c, d); break;
+    }
     dim3 bs(64, 4, 1), gs((g.NZ - 2 + 63) / 64, (g.NY - 2 + 3) / 4, g.NX - 2);
     const int kind = a.rk3 ? a.stage : 3 + a.stage;
     switch (kind) {

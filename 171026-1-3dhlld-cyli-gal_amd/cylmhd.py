@@ -16,11 +16,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcylmhd.so")
 
-LIMITER_MINMOD, LIMITER_MC = 0, 1
+LIMITER_MINMOD, LIMITER_MC, LIMITER_VANLEER, LIMITER_SUPERBEE = 0, 1, 2, 3
 SOLVER_HLLD, SOLVER_HLL = 0, 1
 COOL_NONE, COOL_RAYMOND, COOL_POWERLAW = 0, 1, 2
 GRAV_GALACTIC, GRAV_POINTMASS = 0, 1
 PATH_PLAIN64, PATH_FUSED64, PATH_FUSED32 = 0, 1, 2
+OK, EINVAL, ENOMEM, EHIP, EIO, EUNSUP = 0, -1, -2, -3, -4, -5
 PATH_NAMES = {PATH_PLAIN64: "plain64", PATH_FUSED64: "fused64", PATH_FUSED32: "fused32"}
 
 
@@ -69,7 +70,9 @@ class Diag(C.Structure):
 
 
 class CylmhdError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 _lib = None
@@ -133,7 +136,7 @@ def missing_symbols():
 def check(rc, what=""):
     if rc != 0:
         msg = lib().cylmhd_error_string(rc).decode()
-        raise CylmhdError(f"{what}: {msg} ({rc})")
+        raise CylmhdError(f"{what}: {msg} ({rc})", rc)
 
 
 def params_default():

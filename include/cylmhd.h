@@ -37,6 +37,8 @@ extern "C" {
 /* limiter: firststep.h:37-38 (MinmodLimiterX is the active one)          */
 #define CYLMHD_LIMITER_MINMOD 0   /* limiterx.h:171-217 (MUSCL-Hancock form) */
 #define CYLMHD_LIMITER_MC     1   /* limiterx.h:335-385                      */
+#define CYLMHD_LIMITER_VANLEER  2 /* limiterx.h:266-326 (plain FP64 path)    */
+#define CYLMHD_LIMITER_SUPERBEE 3 /* limiterx.h:394-449 (plain FP64 path)    */
 /* Riemann solver: firststep.h:45-46                                       */
 #define CYLMHD_SOLVER_HLLD    0   /* hlldx.h:28-499                          */
 #define CYLMHD_SOLVER_HLL     1   /* hllx.h:29-184                           */

@@ -31,8 +31,10 @@ GRIDS = {
     "disc16": (16, 16, 10, "disc"),
     "rand16": (16, 8, 10, "rand"),
 }
-SCHEMES = {"mm_hlld": (False, False), "mc_hlld": (True, False),
-           "mm_hll": (False, True), "mc_hll": (True, True)}
+# scheme name -> (limiter define or None, HLL instead of HLLD)
+SCHEMES = {"mm_hlld": (None, False), "mc_hlld": ("REF_LIMITER_MC", False),
+           "mm_hll": (None, True), "mc_hll": ("REF_LIMITER_MC", True),
+           "vl_hlld": ("REF_LIMITER_VL", False), "sb_hlld": ("REF_LIMITER_SB", False)}
 
 
 def hexf(v):
@@ -115,7 +117,7 @@ def main():
     src = os.path.join(HERE, "ref_harness.c")
     jobs = []
     for gname, (nr, nphi, nz, kind) in GRIDS.items():
-        for sname, (mc, hll) in SCHEMES.items():
+        for sname, (limdef, hll) in SCHEMES.items():
             name = f"{gname}_{sname}"
             d = os.path.join(OUT, name)
             os.makedirs(d, exist_ok=True)
@@ -123,8 +125,8 @@ def main():
                 f.write(ref_input_h(grid_values(nr, nphi, nz, kind)))
             cmd = ["gcc", "-O2", "-std=gnu99", "-fopenmp", "-ffp-contract=off", "-w",
                    "-I", d, "-I", REF, src, "-o", os.path.join(d, "ref_harness"), "-lm"]
-            if mc:
-                cmd.insert(1, "-DREF_LIMITER_MC")
+            if limdef:
+                cmd.insert(1, "-D" + limdef)
             if hll:
                 cmd.insert(1, "-DREF_SOLVER_HLL")
             jobs.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE,

@@ -459,9 +459,12 @@ static void glm_source(orc *o, double *dU, double *dV)
 }
 
 /* MinmodLimiter{X,Y,Z} (limiterx.h:171-217, limitery.h:46-94,
- * limiterz.h:146-190) and MCLimiter{X,Y,Z} (limiterx.h:335-385,
- * limitery.h:197-247, limiterz.h:294-358), followed by the interface-state
- * boundaries each of them calls. */
+ * limiterz.h:146-190), MCLimiter{X,Y,Z} (limiterx.h:335-385,
+ * limitery.h:197-247, limiterz.h:294-358), VanLeerLimiter{X,Y,Z}
+ * (limiterx.h:266-326, limitery.h:129-188, limiterz.h:232-285) and
+ * SuperbeeLimiter{X,Y,Z} (limiterx.h:394-449, limitery.h:256-311,
+ * limiterz.h:367-432), each followed by the interface-state boundaries it
+ * calls. */
 static void limiter(orc *o, int dir)
 {
     const cylmhd_params *p = &o->p;
@@ -469,7 +472,7 @@ static void limiter(orc *o, int dir)
     const double *V = o->V;
     double *Vl = o->Vl, *Vr = o->Vr;
     const int NX = o->NX, NY = o->NY, NZ = o->NZ;
-    const int mc = p->limiter == CYLMHD_LIMITER_MC;
+    const int lim = p->limiter;
     const int ilo = dir == 0 ? 1 : 0, jlo = dir == 1 ? 1 : 0, klo = dir == 2 ? 1 : 0;
     const size_t st = dir == 0 ? (size_t)NY * NZ : (dir == 1 ? (size_t)NZ : 1);
     const int nb = 4 + dir, nv = 1 + dir;
@@ -481,8 +484,8 @@ static void limiter(orc *o, int dir)
                 const size_t c = C3(o, i, j, k);
                 const size_t fl = F3(o, i, j, k);
                 const size_t fr = dir == 0 ? F3(o, i - 1, j, k) : (dir == 1 ? F3(o, i, j - 1, k) : F3(o, i, j, k - 1));
-                double lambmax = 0.0, lambmin = 0.0, ol = 0.0, orr = 0.0;
-                if (!mc) {
+                double lambmax = 0.0, lambmin = 0.0;
+                if (lim == CYLMHD_LIMITER_MINMOD) {
                     double b2 = (V[4 * N + c] * V[4 * N + c] + V[5 * N + c] * V[5 * N + c] + V[6 * N + c] * V[6 * N + c]);
                     double gmpr = p->gm * V[7 * N + c];
                     double fast = sqrt(((b2 + gmpr) + sqrt((b2 + gmpr) * (b2 + gmpr) - 4.0 * gmpr * V[nb * N + c] * V[nb * N + c]))
@@ -495,7 +498,7 @@ static void limiter(orc *o, int dir)
                     double a = Vm[c] - Vm[c - st];
                     double b = Vm[c + st] - Vm[c];
                     double grad;
-                    if (mc) {
+                    if (lim == CYLMHD_LIMITER_MC) {
                         double cc = 0.25 * (Vm[c + st] - Vm[c - st]);
                         if (a * b <= 0.0) grad = 0.0;
                         else {
@@ -504,6 +507,24 @@ static void limiter(orc *o, int dir)
                         }
                         Vl[m * NF + fl] = Vm[c] + grad;
                         Vr[m * NF + fr] = Vm[c] - grad;
+                    } else if (lim == CYLMHD_LIMITER_VANLEER) {
+                        if (a * b <= 0.0) grad = 0.0;
+                        else grad = a * b / (a + b);
+                        Vl[m * NF + fl] = Vm[c] + grad;
+                        Vr[m * NF + fr] = Vm[c] - grad;
+                    } else if (lim == CYLMHD_LIMITER_SUPERBEE) {
+                        if (a * b <= 0.0) grad = 0.0;
+                        else {
+                            if (fabs(a) >= fabs(b)) {
+                                if (a > 0.0) grad = rmin(a, 2.0 * b);
+                                else grad = rmax(a, 2.0 * b);
+                            } else {
+                                if (a > 0.0) grad = rmin(2.0 * a, b);
+                                else grad = rmax(2.0 * a, b);
+                            }
+                        }
+                        Vl[m * NF + fl] = Vm[c] + 0.5 * grad;
+                        Vr[m * NF + fr] = Vm[c] - 0.5 * grad;
                     } else {
                         if (a * b <= 0.0) grad = 0.0;
                         else {
@@ -521,29 +542,46 @@ static void limiter(orc *o, int dir)
                             Vr[m * NF + fr] = (Vm[c] - 0.5 * (1.0 + lambmin * o->dt * o->oneodz[k - 1]) * grad);
                         }
                     }
-                    (void)ol; (void)orr;
                 }
             }
         }
     }
     /* interface-state boundaries */
     if (dir == 0) {
-        /* XVlrPeriodicBoundary (xvlrboundary.h:153-180), both limiters */
-        for (int m = 0; m < 9; m++)
-            for (int j = 0; j < NY - 1; j++)
-                for (int k = 0; k < NZ - 1; k++) {
-                    Vl[m * NF + F3(o, 0, j, k)] = Vl[m * NF + F3(o, NX - 2, j, k)];
-                    Vr[m * NF + F3(o, NX - 2, j, k)] = Vr[m * NF + F3(o, 0, j, k)];
-                }
+        if (lim == CYLMHD_LIMITER_VANLEER) {
+            /* XVlrFreeBoundary (xvlrboundary.h:189-216) */
+            for (int m = 0; m < 9; m++)
+                for (int j = 0; j < NY - 1; j++)
+                    for (int k = 0; k < NZ - 1; k++) {
+                        Vl[m * NF + F3(o, 0, j, k)] = Vr[m * NF + F3(o, 0, j, k)];
+                        Vr[m * NF + F3(o, NX - 2, j, k)] = Vl[m * NF + F3(o, NX - 2, j, k)];
+                    }
+        } else {
+            /* XVlrPeriodicBoundary (xvlrboundary.h:153-180) */
+            for (int m = 0; m < 9; m++)
+                for (int j = 0; j < NY - 1; j++)
+                    for (int k = 0; k < NZ - 1; k++) {
+                        Vl[m * NF + F3(o, 0, j, k)] = Vl[m * NF + F3(o, NX - 2, j, k)];
+                        Vr[m * NF + F3(o, NX - 2, j, k)] = Vr[m * NF + F3(o, 0, j, k)];
+                    }
+        }
     } else if (dir == 1) {
-        /* YVlrPeriodicBoundary (yvlrboundary.h:61-87), both limiters */
+        /* YVlrPeriodicBoundary (yvlrboundary.h:61-87), every limiter */
         for (int m = 0; m < 9; m++)
             for (int ii = 0; ii < NX - 1; ii++)
                 for (int k = 0; k < NZ - 1; k++) {
                     Vl[m * NF + F3(o, ii, 0, k)] = Vl[m * NF + F3(o, ii, NY - 2, k)];
                     Vr[m * NF + F3(o, ii, NY - 2, k)] = Vr[m * NF + F3(o, ii, 0, k)];
                 }
-    } else if (!mc) {
+        if (lim == CYLMHD_LIMITER_VANLEER)
+            /* then YVlrInitializeBoundary (yvlrboundary.h:26-52) */
+            for (int m = 0; m < 9; m++)
+                for (int ii = 0; ii < NX - 1; ii++)
+                    for (int k = 0; k < NZ - 1; k++) {
+                        Vl[m * NF + F3(o, ii, 0, k)] = o->Vinit[m * N + C3(o, ii, 0, k)];
+                        Vr[m * NF + F3(o, ii, NY - 2, k)] = o->Vinit[m * N + C3(o, ii, NY - 1, k)];
+                    }
+    } else if (lim == CYLMHD_LIMITER_MINMOD) {
         /* ZLeftVlVzAntisymBzsymBoundary (zvlrboundary.h:63-92) then
          * ZRightVrFreeBoundary (zvlrboundary.h:28-54) */
         for (int ii = 0; ii < NX - 1; ii++)
@@ -554,6 +592,22 @@ static void limiter(orc *o, int dir)
             for (int ii = 0; ii < NX - 1; ii++)
                 for (int j = 0; j < NY - 1; j++)
                     Vr[m * NF + F3(o, ii, j, NZ - 2)] = Vl[m * NF + F3(o, ii, j, NZ - 2)];
+    } else if (lim == CYLMHD_LIMITER_VANLEER) {
+        /* ZVlrFreeBoundary (zvlrboundary.h:137-164) */
+        for (int m = 0; m < 9; m++)
+            for (int ii = 0; ii < NX - 1; ii++)
+                for (int j = 0; j < NY - 1; j++) {
+                    Vl[m * NF + F3(o, ii, j, 0)] = Vr[m * NF + F3(o, ii, j, 0)];
+                    Vr[m * NF + F3(o, ii, j, NZ - 2)] = Vl[m * NF + F3(o, ii, j, NZ - 2)];
+                }
+    } else if (lim == CYLMHD_LIMITER_SUPERBEE) {
+        /* ZVlrPeriodicBoundary (zvlrboundary.h:101-128) */
+        for (int m = 0; m < 9; m++)
+            for (int ii = 0; ii < NX - 1; ii++)
+                for (int j = 0; j < NY - 1; j++) {
+                    Vl[m * NF + F3(o, ii, j, 0)] = Vl[m * NF + F3(o, ii, j, NZ - 2)];
+                    Vr[m * NF + F3(o, ii, j, NZ - 2)] = Vr[m * NF + F3(o, ii, j, 0)];
+                }
     } else {
         /* ZVlrFixedBoundary (zvlrboundary.h:173-200) */
         for (int m = 0; m < 9; m++)

@@ -78,6 +78,16 @@
 #define MinmodLimiterY MCLimiterY
 #define MinmodLimiterZ MCLimiterZ
 #endif
+#ifdef REF_LIMITER_VL
+#define MinmodLimiterX VanLeerLimiterX
+#define MinmodLimiterY VanLeerLimiterY
+#define MinmodLimiterZ VanLeerLimiterZ
+#endif
+#ifdef REF_LIMITER_SB
+#define MinmodLimiterX SuperbeeLimiterX
+#define MinmodLimiterY SuperbeeLimiterY
+#define MinmodLimiterZ SuperbeeLimiterZ
+#endif
 #ifdef REF_SOLVER_HLL
 #define HLLDX HLLX
 #define HLLDY HLLY

@@ -50,7 +50,7 @@ def run_pair(p, Ui, Vi, Uc, Vc, nsteps, path):
     return hist
 
 
-@pytest.mark.parametrize("limiter", [cm.LIMITER_MINMOD, cm.LIMITER_MC])
+@pytest.mark.parametrize("limiter", [cm.LIMITER_MINMOD, cm.LIMITER_MC, cm.LIMITER_VANLEER, cm.LIMITER_SUPERBEE])
 @pytest.mark.parametrize("solver", [cm.SOLVER_HLLD, cm.SOLVER_HLL])
 def test_plain64_disc(gpu, limiter, solver):
     p, Ui, Vi, Uc, Vc = disc_case(2, 16, 16, 10, limiter=limiter, solver=solver)
@@ -64,7 +64,8 @@ def test_plain64_disc(gpu, limiter, solver):
 
 
 @pytest.mark.parametrize("rk", [3, 2])
-@pytest.mark.parametrize("limiter,solver", [(cm.LIMITER_MINMOD, cm.SOLVER_HLLD), (cm.LIMITER_MC, cm.SOLVER_HLL)])
+@pytest.mark.parametrize("limiter,solver", [(cm.LIMITER_MINMOD, cm.SOLVER_HLLD), (cm.LIMITER_MC, cm.SOLVER_HLL),
+                                             (cm.LIMITER_VANLEER, cm.SOLVER_HLLD), (cm.LIMITER_SUPERBEE, cm.SOLVER_HLLD)])
 def test_plain64_random_state(gpu, rk, limiter, solver):
     p, Ui, Vi, Uc, Vc = rand_case(rk=rk, limiter=limiter, solver=solver)
     hist = run_pair(p, Ui, Vi, Uc, Vc, 3, cm.PATH_PLAIN64)
@@ -122,3 +123,14 @@ def test_totals_100_steps(gpu, path):
     for key in ("mass", "energy", "divb_l1"):
         assert abs(dg[key] - do[key]) <= TOL_INT * abs(do[key]) + 1e-300, (key, dg[key], do[key])
     g.close()
+
+
+@pytest.mark.parametrize("path", [cm.PATH_FUSED64, cm.PATH_FUSED32])
+def test_fused_rejects_vanleer_superbee(gpu, path):
+    for lim in (cm.LIMITER_VANLEER, cm.LIMITER_SUPERBEE):
+        p = cm.params_config(2, 16, 16, 10)
+        p.limiter = lim
+        p.path = path
+        with pytest.raises(cm.CylmhdError) as e:
+            cm.Context(p)
+        assert e.value.code == cm.EUNSUP

@@ -18,7 +18,8 @@ from cases import disc_case, rand_case
 from oracle_py import Oracle, parse_counts, parse_dt_mesh, ref_harness, run_ref
 
 SCHEMES = {"mm_hlld": (cm.LIMITER_MINMOD, cm.SOLVER_HLLD), "mc_hlld": (cm.LIMITER_MC, cm.SOLVER_HLLD),
-           "mm_hll": (cm.LIMITER_MINMOD, cm.SOLVER_HLL), "mc_hll": (cm.LIMITER_MC, cm.SOLVER_HLL)}
+           "mm_hll": (cm.LIMITER_MINMOD, cm.SOLVER_HLL), "mc_hll": (cm.LIMITER_MC, cm.SOLVER_HLL),
+           "vl_hlld": (cm.LIMITER_VANLEER, cm.SOLVER_HLLD), "sb_hlld": (cm.LIMITER_SUPERBEE, cm.SOLVER_HLLD)}
 
 
 def run_oracle(p, Ui, Vi, Uc, Vc, nsteps):
