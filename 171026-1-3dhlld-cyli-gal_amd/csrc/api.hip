@@ -334,7 +334,7 @@ int cylmhd_download_state(cylmhd_ctx *c, double *U, double *V)
     } else {
         /* U (FP64 copy of the path's state) -> fs.u64_scratch, V = UtoV(U) -> ds.V */
         Consts k = make_consts(c, 0.0, 0.0);
-        int rc = fused_export(c->stream, c->fs, c->dg, c->ds, k, c->ds.V, nullptr);
+        int rc = fused_export(c->stream, c->fs, c->dg, c->ds, k, c->ds.V);
         if (rc) return rc;
         if (U) HIPCHK(hipMemcpyAsync(U, c->fs.u64_scratch, bytes, hipMemcpyDeviceToHost, c->stream));
         if (V) HIPCHK(hipMemcpyAsync(V, c->ds.V, bytes, hipMemcpyDeviceToHost, c->stream));

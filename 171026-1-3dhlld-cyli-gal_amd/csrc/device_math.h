@@ -76,6 +76,17 @@ __device__ __forceinline__ T cooling(const Consts &c, T rho, T pr)
     return T(0.0);
 }
 
+/* (b2 + gp)^2 - 4 gp bn^2 >= (b2 - gp)^2 >= 0.  With FP32 state the two
+ * products may be fused into an FMA, which can leave it a few ulps below
+ * zero when b2 = bn^2 and b2 ~ gp (a purely normal field at beta ~ 1.2);
+ * that case is clamped at 0.  FP64 keeps the reference's unfused form. */
+template <typename T>
+__device__ __forceinline__ T fast_disc(T b2, T gmpr, T bn)
+{
+    const T d = (b2 + gmpr) * (b2 + gmpr) - T(4.0) * gmpr * bn * bn;
+    return sizeof(T) == 4 ? fmaxf(float(d), 0.0f) : d;
+}
+
 /* ---- limiter interface states ------------------------------------------
  * upper = state at the cell's +side face (Vl of that face),
  * lower = state at the cell's -side face (Vr of that face).
@@ -94,12 +105,39 @@ __device__ __forceinline__ void cell_states(const Consts &c, const T *vm, const 
         const int nb = 4 + DIR, nv = 1 + DIR;
         const T b2 = (v0[4] * v0[4] + v0[5] * v0[5] + v0[6] * v0[6]);
         const T gmpr = T(c.gm) * v0[7];
-        const T fast = sqrt(((b2 + gmpr) + sqrt((b2 + gmpr) * (b2 + gmpr) - T(4.0) * gmpr * v0[nb] * v0[nb]))
-                            / (T(2.0) * v0[0]));
+        const T fast = sqrt(((b2 + gmpr) + sqrt(fast_disc(b2, gmpr, v0[nb]))) / (T(2.0) * v0[0]));
         lambmax = rmax(T(0), v0[nv] + fast);
         lambmin = rmin(T(0), v0[nv] - fast);
     }
     const T dt = T(c.dt);
+    if (sizeof(T) == 4 && (LIM == 0 || LIM == 1)) {
+        /* FP32 state: the same slopes as three-input min/max, which gfx950
+         * issues as one v_min3/v_max3 each.  For a > 0 the limited slope is
+         * max(0, min(a, b[, c])) and for a <= 0 min(0, max(a, b[, c])): with
+         * a b <= 0 both give 0, otherwise the reference's min/max chain
+         * (the MC centred slope c has the sign of a and b when they agree).
+         * Min and max round nothing, so the result equals the reference
+         * form except where a*b underflows. */
+        const T hu = T(0.5) * (T(1.0) - lambmax * dt * f1hi * f2);
+        const T hl = T(0.5) * (T(1.0) + lambmin * dt * f1lo * f2);
+#pragma unroll
+        for (int m = 0; m < 9; m++) {
+            const T a = v0[m] - vm[m];
+            const T b = vp[m] - v0[m];
+            T grad;
+            if (LIM == 1) {
+                const T cc = T(0.25) * (vp[m] - vm[m]);
+                grad = a > T(0.0) ? fmaxf(0.0f, fminf(fminf(a, b), cc)) : fminf(0.0f, fmaxf(fmaxf(a, b), cc));
+                if (upper) upper[m] = v0[m] + grad;
+                if (lower) lower[m] = v0[m] - grad;
+            } else {
+                grad = a > T(0.0) ? fmaxf(0.0f, fminf(a, b)) : fminf(0.0f, fmaxf(a, b));
+                if (upper) upper[m] = v0[m] + hu * grad;
+                if (lower) lower[m] = v0[m] - hl * grad;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int m = 0; m < 9; m++) {
         const T a = v0[m] - vm[m];
@@ -231,7 +269,7 @@ template <typename T>
 __device__ __forceinline__ T fast_speed(T gm, const T *V, int nb, T b2)
 {
     const T gmpr = gm * V[7];
-    return sqrt(((b2 + gmpr) + sqrt((b2 + gmpr) * (b2 + gmpr) - T(4.0) * gmpr * V[nb] * V[nb])) / (T(2.0) * V[0]));
+    return sqrt(((b2 + gmpr) + sqrt(fast_disc(b2, gmpr, V[nb]))) / (T(2.0) * V[0]));
 }
 
 /* ---- HLLD (hlldx.h:53-477, hlldy.h:55-485, hlldz.h:53-481) ------------- */

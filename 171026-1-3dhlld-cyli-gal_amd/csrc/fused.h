@@ -24,20 +24,63 @@ struct FusedState {
     int dt_blocks = 0;
     bool dt_valid = false;
     int nseg = 1, seglen = 1, tiles_j = 1, tiles_k = 1;
+    /* geometry packed for the stage kernel (few base pointers):
+     *   itab [NX][IT] FP64  x, xm, oneox, oneodx, xabs, spiral r factors
+     *   ikg  [NX*NZ][4] T   g1, g2, g3, g5
+     *   ikd  [NX*NZ][4] FP64 minlength, Grxaxi, Grzaxi, spiral q
+     *   ktab [NZ][KT] FP64  oneodz, z, spiral z factors, zabs */
+    double *itab = nullptr, *ikd = nullptr, *ktab = nullptr;
+    void *ikg = nullptr;
 };
 
 using AllocFn = std::function<int(size_t, void **)>;
 
-int fused_alloc(FusedState &fs, const DevGeom &g, const DevState &d, bool fp32, AllocFn alloc);
+/* one set of entry points per state precision (fused.hip built twice) */
+#define CYL_FUSED_DECLS                                                                                   \
+    int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc);                       \
+    int set_rhofloor(hipStream_t s, FusedState &fs, const DevState &d, double rhocr);                     \
+    int load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const double *U64); \
+    int export_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,  \
+                     double *V64);                                                                        \
+    int setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c);       \
+    int stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,        \
+              const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1);                                        \
+    int diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d);
+namespace f32 { CYL_FUSED_DECLS }
+namespace f64 { CYL_FUSED_DECLS }
+#undef CYL_FUSED_DECLS
+
+inline int fused_alloc(FusedState &fs, const DevGeom &g, const DevState &d, bool fp32, AllocFn a)
+{
+    return fp32 ? f32::alloc(fs, g, d, a) : f64::alloc(fs, g, d, a);
+}
 /* rhofloor = rhocr * Uinitial[0] (saveinitial.h:44), from d.Uinit */
-int fused_set_rhofloor(hipStream_t s, FusedState &fs, const DevState &d, double rhocr);
-int fused_load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const double *U64);
-int fused_export(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
-                 double *V64, const double *unused);
-int fused_setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c);
-int fused_stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
-                const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1);
-int fused_diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d);
+inline int fused_set_rhofloor(hipStream_t s, FusedState &fs, const DevState &d, double rhocr)
+{
+    return fs.fp32 ? f32::set_rhofloor(s, fs, d, rhocr) : f64::set_rhofloor(s, fs, d, rhocr);
+}
+inline int fused_load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const double *U64)
+{
+    return fs.fp32 ? f32::load_state(s, fs, g, d, U64) : f64::load_state(s, fs, g, d, U64);
+}
+inline int fused_export(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
+                        double *V64)
+{
+    return fs.fp32 ? f32::export_state(s, fs, g, d, c, V64) : f64::export_state(s, fs, g, d, c, V64);
+}
+inline int fused_setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c)
+{
+    return fs.fp32 ? f32::setdt(s, fs, g, d, c) : f64::setdt(s, fs, g, d, c);
+}
+inline int fused_stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
+                       const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1)
+{
+    return fs.fp32 ? f32::stage(s, fs, g, d, c, a, ev0, ev1) : f64::stage(s, fs, g, d, c, a, ev0, ev1);
+}
+inline int fused_diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d)
+{
+    return fs.fp32 ? f32::diag(s, fs, g, d) : f64::diag(s, fs, g, d);
+}
 
 }  // namespace cyl
 #endif

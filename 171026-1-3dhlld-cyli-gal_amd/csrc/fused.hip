@@ -36,7 +36,27 @@
 
 #include <algorithm>
 
+/* This file is compiled twice: FUSED_PREC=64 (FP64 state, -ffp-contract=off
+ * like the plain path) and FUSED_PREC=32 (FP32 state; contraction into FMAs
+ * and the fast FP32 divide/sqrt are allowed there, its bar is a relative L1
+ * error, not bit parity).  Each build defines the entry points of its own
+ * namespace, cyl::f64 or cyl::f32. */
+#if FUSED_PREC == 32
+#define FUSED_NS f32
+#elif FUSED_PREC == 64
+#define FUSED_NS f64
+#else
+#error "build with -DFUSED_PREC=32 or -DFUSED_PREC=64"
+#endif
+
 namespace cyl {
+namespace FUSED_NS {
+
+#if FUSED_PREC == 32
+using FT = float;
+#else
+using FT = double;
+#endif
 
 #define C3(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * g.NZ + (size_t)(k))
 
@@ -105,9 +125,15 @@ template <> struct Prec<float> {
     }
 };
 
+/* packed geometry rows (FusedState::itab, ktab) */
+enum { IT_X, IT_XM, IT_OX, IT_ODX, IT_XABS, IT_X2S2, IT_O1X2S2, IT_XS2O15, IT_X2S2O15, IT = 12 };
+enum { KT_ODZ, KT_Z, KT_SZS, KT_ONEPA, KT_OZ2S2, KT_ZABS, KT = 8 };
+
 template <typename T> struct FArgs {
     DevGeom g;
     Consts c;
+    const double *itab, *ikd, *ktab;
+    const T *ikg;
     const T *Uin, *Un;
     T *Uout, *shadow;
     const T *rhofloor;
@@ -136,7 +162,7 @@ __device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, T *v
 #pragma unroll
     for (int m = 0; m < 9; m++) u[m] = a.Uin[m * a.N + cc];
     u[8] = T(a.c.psicoef) * u[8];
-    Prec<T>::utov(a.c, u, a.g.oneox[i], v);
+    Prec<T>::utov(a.c, u, a.itab[(size_t)i * IT + IT_OX], v);
 }
 
 /* XCD-aware block order: consecutive tiles (which share halo planes) go to
@@ -148,24 +174,97 @@ __device__ __forceinline__ int remap_block(int b, int nb)
     return (b % 8) * per + b / 8;
 }
 
-/* Tile shapes: FP32 15 x 16 cells / 256 threads, FP64 7 x 16 / 128 threads
- * (the FP64 LDS ring is twice as large).  NT = number of phi faces. */
-template <typename T> struct Tile;
-template <> struct Tile<float>  { static constexpr int TJ = 15, TK = 16; };
-template <> struct Tile<double> { static constexpr int TJ = 7, TK = 16; };
+/* ---- packed geometry ---------------------------------------------------- */
+template <typename T>
+__global__ void k_pack(DevGeom g, double *itab, T *ikg, double *ikd, double *ktab)
+{
+    const size_t nik = (size_t)g.NX * g.NZ;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < nik; q += (size_t)gridDim.x * blockDim.x) {
+        ikg[4 * q + 0] = T(g.g1[q]); ikg[4 * q + 1] = T(g.g2[q]); ikg[4 * q + 2] = T(g.g3[q]); ikg[4 * q + 3] = T(g.g5[q]);
+        ikd[4 * q + 0] = g.minlength[q]; ikd[4 * q + 1] = g.grxaxi[q]; ikd[4 * q + 2] = g.grzaxi[q]; ikd[4 * q + 3] = g.sp_q[q];
+        if (q < (size_t)g.NX) {
+            const int i = (int)q;
+            double *r = itab + (size_t)i * IT;
+            r[IT_X] = g.x[i]; r[IT_XM] = g.xm[i]; r[IT_OX] = g.oneox[i]; r[IT_ODX] = g.oneodx[i];
+            r[IT_XABS] = i >= g.NX - 11 ? g.xabs[i - (g.NX - 11)] : 0.0;
+            r[IT_X2S2] = g.sp_x2s2[i]; r[IT_O1X2S2] = g.sp_o1x2s2[i];
+            r[IT_XS2O15] = g.sp_xs2o15[i]; r[IT_X2S2O15] = g.sp_x2s2o15[i];
+        }
+        if (q < (size_t)g.NZ) {
+            const int k = (int)q;
+            double *r = ktab + (size_t)k * KT;
+            r[KT_ODZ] = g.oneodz[k]; r[KT_Z] = g.z[k]; r[KT_SZS] = g.sp_szs[k];
+            r[KT_ONEPA] = g.sp_onepa[k]; r[KT_OZ2S2] = g.sp_oz2s2[k];
+            r[KT_ZABS] = k >= g.NZ - 11 ? g.zabs[k - (g.NZ - 11)] : 0.0;
+        }
+    }
+}
 
+/* gravity_cell (ctx.h) from the packed rows, same expression order */
+template <typename T>
+__device__ __forceinline__ bool gravity_packed(const FArgs<T> &a, int i, int j, const double *it, const double *kt,
+                                               const double *ikd, double &gx, double &gy, double &gz)
+{
+    const DevGeom &g = a.g;
+    if (i < g.irin) {
+        gx = ikd[1];
+        gy = 0.0;
+        gz = ikd[2];
+        return false;
+    }
+    const double m1 = 10.0;
+    const double epssp = g.epssp;
+    const double q = ikd[3];
+    const double szs = kt[KT_SZS];
+    const double sinphase = g.sincos[(size_t)i * g.NY + j];
+    const double cosphase = g.sincos[(size_t)g.NX * g.NY + (size_t)i * g.NY + j];
+    const double x = it[IT_X], z = kt[KT_Z];
+    const double spiralfr = (-epssp * m1 * q * szs * it[IT_XS2O15]
+                             * ((x * q * q - ((2.0 - it[IT_X2S2]) * it[IT_O1X2S2])) * cosphase
+                                + g.armnum * g.cotpitch * sinphase));
+    const double spiralfphi = (epssp * m1 * q * szs * it[IT_XS2O15] * g.armnum * sinphase);
+    const double spiralfz = (-epssp * m1 * q * z * szs * it[IT_X2S2O15]
+                             * (kt[KT_ONEPA] * q * q + kt[KT_OZ2S2]) * cosphase);
+    gx = ikd[1] + spiralfr;
+    gy = 0.0 + spiralfphi;
+    gz = ikd[2] + spiralfz;
+    return true;
+}
+
+/* Tile shapes: a block owns TJ x TK (phi x z) cells and has (TJ+2) x TK
+ * threads, one per phi-state task (the TJ rows plus one halo row each
+ * side): FP32 14 x 16 / 256 threads, FP64 6 x 16 / 128 threads.  The LDS
+ * of one block (3-plane ring + phi and z state buffers) is 75 KB (FP32) or
+ * 77 KB (FP64), so two blocks share a CU. */
+template <typename T> struct Tile;
+template <> struct Tile<float>  { static constexpr int TJ = 14, TK = 16, WAVES = 2; };
+template <> struct Tile<double> { static constexpr int TJ = 6, TK = 16, WAVES = 1; };
+
+/* WAVES: waves per SIMD the register allocation must allow (two blocks of
+ * four waves per CU for FP32; FP64 is held to two blocks of two waves by
+ * its LDS). */
 template <typename T, bool MC, bool HLL>
-__global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs<T> a)
+__global__ __launch_bounds__((Tile<T>::TJ + 2) * Tile<T>::TK, Tile<T>::WAVES) void k_fused(FArgs<T> a)
 {
     constexpr int TJ = Tile<T>::TJ, TK = Tile<T>::TK;
-    constexpr int NT = (TJ + 1) * TK, NCELL = TJ * TK, NHALO = 4 * TK + 4 * TJ;
+    constexpr int NTH = (TJ + 2) * TK;           /* threads = phi-state tasks */
+    constexpr int NZT = TJ * (TK + 2);           /* z-state tasks */
+    constexpr int NCELL = TJ * TK, NHALO = 4 * TK + 4 * TJ;
+    static_assert(NZT <= NTH && TJ * (TK + 1) <= NTH && NHALO <= NTH, "tile does not fit the block");
+    constexpr int LIM = MC ? 1 : 0;
     /* padded plane: (TJ+4) x (TK+4) cells (corners unused), 9 components,
      * 3 ring slots; every stencil read is a per-thread base + an immediate */
     constexpr int PK = TK + 4, PS = (TJ + 4) * PK, SLOT = 9 * PS;
     const DevGeom &g = a.g;
     const Consts &c = a.c;
     __shared__ T ring[3 * SLOT];
-    __shared__ T fbuf[9 * NT];
+    /* limiter states of plane i: phi direction, task q = (row + 1) * TK + col
+     * for rows -1..TJ; z direction, task q = row * (TK + 2) + col + 1 for
+     * cols -1..TK.  [0] upper (+face) state, [1] lower (-face) state; the
+     * face pass overwrites a cell's lower state with the flux through its
+     * -face (only that face reads it). */
+    __shared__ __align__(16) T sy[2][9 * NTH];
+    __shared__ T sz[2][9 * NZT];
 
     const int t = threadIdx.x;
     const int b = remap_block(blockIdx.x, gridDim.x);
@@ -184,6 +283,21 @@ __global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs
     const bool valid = is_cell && gj <= g.NY - 2 && gk <= g.NZ - 2;
     const size_t N = a.N;
     const int cbase = (cj + 2) * PK + ck + 2;           /* own cell in a padded plane */
+    /* phi-state task: row yr (-1..TJ), col yc */
+    const int yr = t / TK - 1, yc = t % TK;
+    const int ybase = (yr + 1) * PK + yc + 2;           /* rows yr-1..yr+1 of col yc */
+    /* z-state task: row zr, col zc (-1..TK) */
+    const bool zstate = t < NZT;
+    const int zr = t / (TK + 2), zc = t % (TK + 2) - 1;
+    const int zbase = (zr + 2) * PK + zc + 1;           /* cols zc-1..zc+1 of row zr */
+    const int zcg = kmap(g, k0 + zc);
+    /* faces: phi face (fyj, fyk) between rows fyj-1 and fyj; z face
+     * (fzj, fzk) between cols fzk-1 and fzk, global face index kf */
+    const bool yface = t < (TJ + 1) * TK;
+    const int fyj = t / TK, fyk = t % TK;
+    const bool zface = t < TJ * (TK + 1);
+    const int fzj = t / (TK + 1), fzk = t % (TK + 1);
+    const int kf = k0 + fzk - 1;
 
     auto ld9 = [&](int off, T *v) {
 #pragma unroll
@@ -226,7 +340,7 @@ __global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs
         if (is_cell) {
             T vm[9], v0[9], vp[9];
             ld9(0 * SLOT + cbase, vm); ld9(1 * SLOT + cbase, v0); ld9(2 * SLOT + cbase, vp);
-            cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[1]), T(g.oneodx[0]), T(1.0), up_prev, (T *)nullptr);
+            cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[1 * IT + IT_ODX]), T(a.itab[0 * IT + IT_ODX]), T(1.0), up_prev, (T *)nullptr);
             const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
 #pragma unroll
             for (int m = 0; m < 9; m++) fx_prev[m] = a.edge[m * NE + e];   /* face 0 */
@@ -239,7 +353,7 @@ __global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs
             T vm[9], v0[9], vp[9];
             ld9(((ibeg - 2) % 3) * SLOT + cbase, vm); ld9(((ibeg - 1) % 3) * SLOT + cbase, v0);
             ld9((ibeg % 3) * SLOT + cbase, vp);
-            cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[ibeg - 1]), T(g.oneodx[ibeg - 2]), T(1.0), upm, (T *)nullptr);
+            cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[(ibeg - 1) * IT + IT_ODX]), T(a.itab[(ibeg - 2) * IT + IT_ODX]), T(1.0), upm, (T *)nullptr);
         }
         __syncthreads();
         load_center(ibeg + 1);
@@ -248,8 +362,8 @@ __global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs
             T vm[9], v0[9], vp[9], lo[9];
             ld9(((ibeg - 1) % 3) * SLOT + cbase, vm); ld9((ibeg % 3) * SLOT + cbase, v0);
             ld9(((ibeg + 1) % 3) * SLOT + cbase, vp);
-            cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[ibeg]), T(g.oneodx[ibeg - 1]), T(1.0), up_prev, lo);
-            riemann<0, HLL>(c, upm, lo, T(g.xm[ibeg - 1]), T(g.oneox[ibeg - 1]), fx_prev);
+            cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[ibeg * IT + IT_ODX]), T(a.itab[(ibeg - 1) * IT + IT_ODX]), T(1.0), up_prev, lo);
+            riemann<0, HLL>(c, upm, lo, T(a.itab[(ibeg - 1) * IT + IT_XM]), T(a.itab[(ibeg - 1) * IT + IT_OX]), fx_prev);
         }
     }
 
@@ -259,7 +373,8 @@ __global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs
         load_halo(i);
         __syncthreads();
         const int s0 = (i % 3) * SLOT, s1 = ((i + 1) % 3) * SLOT, s2 = ((i + 2) % 3) * SLOT;
-        /* ---- x: flux at i+1/2 ------------------------------------------- */
+        const double *it = a.itab + (size_t)i * IT;
+        /* ---- x: flux at i+1/2, divergence term of the r faces ------------ */
         if (is_cell) {
             T fxn[9];
             if (i == g.NX - 2) {
@@ -269,93 +384,107 @@ __global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs
             } else {
                 T vm[9], v0[9], vp[9], lo[9], up[9];
                 ld9(s0 + cbase, vm); ld9(s1 + cbase, v0); ld9(s2 + cbase, vp);
-                cell_states<0, MC>(c, vm, v0, vp, T(g.oneodx[i + 1]), T(g.oneodx[i]), T(1.0), up, lo);
-                riemann<0, HLL>(c, up_prev, lo, T(g.xm[i]), T(g.oneox[i]), fxn);
+                cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[(i + 1) * IT + IT_ODX]), T(it[IT_ODX]), T(1.0), up, lo);
+                riemann<0, HLL>(c, up_prev, lo, T(it[IT_XM]), T(it[IT_OX]), fxn);
 #pragma unroll
                 for (int m = 0; m < 9; m++) up_prev[m] = up[m];
             }
             const size_t ik = (size_t)i * g.NZ + kmap(g, gk);
-            const T g1 = T(g.g1[ik]), g2 = T(g.g2[ik]);
+            const T g1 = a.ikg[4 * ik + 0], g2 = a.ikg[4 * ik + 1];
 #pragma unroll
             for (int m = 0; m < 9; m++) { acc[m] = g2 * fxn[m] - g1 * fx_prev[m]; fx_prev[m] = fxn[m]; }
         }
-        /* ---- y: (TJ+1) x TK faces --------------------------------------- */
+        /* ---- limiter states, once per cell and direction ---------------- */
         {
-            const int fj = t / TK, fk = t % TK;          /* face between rows fj-1 and fj */
-            const int yb = s0 + fj * PK + fk + 2;        /* row fj-2 of column fk */
-            T vm[9], v0[9], vp[9], vl[9], vr[9], fy[9];
-            const T f1 = T(g.oneox[i]), f2 = T(g.oneody);
-            ld9(yb, vm); ld9(yb + PK, v0); ld9(yb + 2 * PK, vp);
-            cell_states<1, MC>(c, vm, v0, vp, f1, f1, f2, vl, (T *)nullptr);
-            ld9(yb + 3 * PK, vm);                        /* rows fj-1, fj, fj+1 */
-            cell_states<1, MC>(c, v0, vp, vm, f1, f1, f2, (T *)nullptr, vr);
-            riemann<1, HLL>(c, vl, vr, T(g.x[i]), T(g.oneox[i]), fy);
+            T vm[9], v0[9], vp[9], up[9], lo[9];
+            const T f1 = T(it[IT_OX]), f2 = T(g.oneody);
+            ld9(s0 + ybase, vm); ld9(s0 + ybase + PK, v0); ld9(s0 + ybase + 2 * PK, vp);
+            cell_states<1, LIM>(c, vm, v0, vp, f1, f1, f2, up, lo);
 #pragma unroll
-            for (int m = 0; m < 9; m++) fbuf[m * NT + t] = fy[m];
+            for (int m = 0; m < 9; m++) { sy[0][m * NTH + t] = up[m]; sy[1][m * NTH + t] = lo[m]; }
+        }
+        if (zstate) {
+            T vm[9], v0[9], vp[9], up[9], lo[9];
+            ld9(s0 + zbase, vm); ld9(s0 + zbase + 1, v0); ld9(s0 + zbase + 2, vp);
+            cell_states<2, LIM>(c, vm, v0, vp, T(a.ktab[zcg * KT + KT_ODZ]), T(a.ktab[(zcg > 0 ? zcg - 1 : 0) * KT + KT_ODZ]), T(1.0), up, lo);
+#pragma unroll
+            for (int m = 0; m < 9; m++) { sz[0][m * NZT + t] = up[m]; sz[1][m * NZT + t] = lo[m]; }
         }
         __syncthreads();
-        if (is_cell) {
-            const size_t ik = (size_t)i * g.NZ + kmap(g, gk);
-            const T g3 = T(g.g3[ik]), g4 = g3;
+        /* ---- phi faces: (TJ+1) x TK ------------------------------------- */
+        if (yface) {
+            T vl[9], vr[9], fy[9];
+            const int qu = fyj * TK + fyk, ql = (fyj + 1) * TK + fyk;
 #pragma unroll
-            for (int m = 0; m < 9; m++)
-                acc[m] = acc[m] + (g4 * fbuf[m * NT + (cj + 1) * TK + ck] - g3 * fbuf[m * NT + cj * TK + ck]);
+            for (int m = 0; m < 9; m++) { vl[m] = sy[0][m * NTH + qu]; vr[m] = sy[1][m * NTH + ql]; }
+            riemann<1, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fy);
+#pragma unroll
+            for (int m = 0; m < 9; m++) sy[1][m * NTH + ql] = fy[m];
         }
-        __syncthreads();
-        /* ---- z: TJ x (TK+1) faces --------------------------------------- */
-        if (t < TJ * (TK + 1)) {
-            const int fj = t / (TK + 1), fk = t % (TK + 1);   /* face between cols fk-1 and fk */
-            const int kf = k0 + fk - 1;                       /* global face index */
-            const int jgl = jmap(g, j0 + fj);
-            const int zb = s0 + (fj + 2) * PK + fk;           /* col fk-2 of row fj */
-            T vm[9], v0[9], vp[9], vl[9], vr[9], fz[9];
+        /* ---- z faces: TJ x (TK+1) --------------------------------------- */
+        if (zface && kf <= g.NZ - 2) {
+            T vl[9], vr[9], fz[9];
+            const int qu = fzj * (TK + 2) + fzk, ql = qu + 1;
+            const int jgl = jmap(g, j0 + fzj);
             if (kf == 0) {
                 if (MC) {
+                    /* ZVlrFixedBoundary: Vinitial at k = 0 */
                     const size_t NP = (size_t)g.NX * g.NY;
 #pragma unroll
                     for (int m = 0; m < 9; m++) vl[m] = T(a.Vinit_z[(size_t)m * 2 * NP + (size_t)i * g.NY + jgl]);
                 } else {
-                    /* lower state of cell 2 (cols fk..fk+2), vz negated */
-                    ld9(zb + 2, vm); ld9(zb + 3, v0); ld9(zb + 4, vp);
-                    cell_states<2, MC>(c, vm, v0, vp, T(g.oneodz[2]), T(g.oneodz[1]), T(1.0), (T *)nullptr, vl);
+                    /* ZLeftVlVzAntisymBzsym: lower state of cell 2, vz negated */
+                    T vm[9], v0[9], vp[9];
+                    const int zb = s0 + (fzj + 2) * PK + 2;       /* cols 0..2 of the tile = k 1..3 */
+                    ld9(zb, vm); ld9(zb + 1, v0); ld9(zb + 2, vp);
+                    cell_states<2, LIM>(c, vm, v0, vp, T(a.ktab[2 * KT + KT_ODZ]), T(a.ktab[1 * KT + KT_ODZ]), T(1.0), (T *)nullptr, vl);
                     vl[3] = -vl[3];
                 }
             } else {
-                const int kc = kmap(g, kf);
-                ld9(zb, vm); ld9(zb + 1, v0); ld9(zb + 2, vp);
-                cell_states<2, MC>(c, vm, v0, vp, T(g.oneodz[kc]), T(g.oneodz[kc > 0 ? kc - 1 : 0]), T(1.0), vl, (T *)nullptr);
+#pragma unroll
+                for (int m = 0; m < 9; m++) vl[m] = sz[0][m * NZT + qu];
             }
             if (kf == g.NZ - 2) {
                 if (MC) {
+                    /* ZVlrFixedBoundary: Vinitial at k = NZ-1 */
                     const size_t NP = (size_t)g.NX * g.NY;
 #pragma unroll
                     for (int m = 0; m < 9; m++) vr[m] = T(a.Vinit_z[(size_t)m * 2 * NP + NP + (size_t)i * g.NY + jgl]);
                 } else {
+                    /* ZRightVrFreeBoundary: Vr = Vl */
 #pragma unroll
                     for (int m = 0; m < 9; m++) vr[m] = vl[m];
                 }
             } else {
-                const int kc = kmap(g, kf + 1);
-                ld9(zb + 1, vm); ld9(zb + 2, v0); ld9(zb + 3, vp);
-                cell_states<2, MC>(c, vm, v0, vp, T(g.oneodz[kc]), T(g.oneodz[kc > 0 ? kc - 1 : 0]), T(1.0), (T *)nullptr, vr);
-            }
-            riemann<2, HLL>(c, vl, vr, T(g.x[i]), T(g.oneox[i]), fz);
 #pragma unroll
-            for (int m = 0; m < 9; m++) fbuf[m * NT + t] = fz[m];
+                for (int m = 0; m < 9; m++) vr[m] = sz[1][m * NZT + ql];
+            }
+            riemann<2, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fz);
+#pragma unroll
+            for (int m = 0; m < 9; m++) sz[1][m * NZT + ql] = fz[m];
         }
         __syncthreads();
         /* ---- divergence, sources, stage update, absorb, Check, store ----- */
         if (valid) {
             const size_t ik = (size_t)i * g.NZ + gk;
-            const T g5 = T(g.g5[ik]), g6 = g5;
+            {
+                const T g3 = a.ikg[4 * ik + 2], g4 = g3;
+                const int qm = (cj + 1) * TK + ck, qp = qm + TK;
 #pragma unroll
-            for (int m = 0; m < 9; m++)
-                acc[m] = acc[m] + (g6 * fbuf[m * NT + cj * (TK + 1) + ck + 1] - g5 * fbuf[m * NT + cj * (TK + 1) + ck]);
+                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g4 * sy[1][m * NTH + qp] - g3 * sy[1][m * NTH + qm]);
+            }
+            {
+                const T g5 = a.ikg[4 * ik + 3], g6 = g5;
+                const int qm = cj * (TK + 2) + ck + 1, qp = qm + 1;
+#pragma unroll
+                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz[1][m * NZT + qp] - g5 * sz[1][m * NZT + qm]);
+            }
             T v[9], S[9], u[9], un[9];
             ld9(s0 + cbase, v);
+            const double *kt = a.ktab + (size_t)gk * KT;
             double gx, gy, gz;
-            const bool spiral = gravity_cell(g, i, gj, gk, gx, gy, gz);
-            sources(c, v, T(g.x[i]), T(g.oneox[i]), T(gx), T(gy), T(gz), spiral, S);
+            const bool spiral = gravity_packed(a, i, gj, it, kt, a.ikd + 4 * ik, gx, gy, gz);
+            sources(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
             const size_t cc = C3(i, gj, gk);
 #pragma unroll
             for (int m = 0; m < 9; m++) u[m] = a.Uin[m * N + cc];
@@ -386,14 +515,14 @@ __global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs
 #pragma unroll
                     for (int m = 0; m < 9; m++) {
                         const T ui = T(a.Uinit[m * N + cc]);
-                        if (ax) u[m] -= (T(g.xabs[i - (g.NX - 11)]) * (u[m] - ui));
+                        if (ax) u[m] -= (T(it[IT_XABS]) * (u[m] - ui));
                         if (ac) u[m] -= (T(g.cabs[(size_t)i * g.NZ + gk]) * (u[m] - ui));
-                        if (az) u[m] -= (T(g.zabs[gk - (g.NZ - 11)]) * (u[m] - ui));
+                        if (az) u[m] -= (T(kt[KT_ZABS]) * (u[m] - ui));
                     }
                 }
             }
-            Prec<T>::utov(c, u, g.oneox[i], v);
-            const int fl = Prec<T>::check(c, u, v, g.x[i], a.rhofloor[cc]);
+            Prec<T>::utov(c, u, it[IT_OX], v);
+            const int fl = Prec<T>::check(c, u, v, it[IT_X], a.rhofloor[cc]);
             f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
 #pragma unroll
             for (int m = 0; m < 9; m++) a.Uout[m * N + cc] = u[m];
@@ -402,7 +531,7 @@ __global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs
                 const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
                 const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
                 const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
-                const double t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / g.minlength[ik];
+                const double t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / a.ikd[4 * ik + 0];
                 const double tc = (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7)));
                 const double tt = rmax(t1, tc);
                 const long long ix = (long long)cc;
@@ -421,11 +550,11 @@ __global__ __launch_bounds__((Tile<T>::TJ + 1) * Tile<T>::TK) void k_fused(FArgs
     }
     if (a.want_dt) {
         __syncthreads();
-        double *sv = (double *)&fbuf[0];
-        long long *sx = (long long *)(sv + NT);
+        double *sv = (double *)&sy[0][0];
+        long long *sx = (long long *)(sv + NTH);
         sv[t] = best; sx[t] = bix;
         __syncthreads();
-        for (int off = NT / 2; off > 0; off >>= 1) {
+        for (int off = NTH / 2; off > 0; off >>= 1) {
             if (t < off) {
                 const double v = sv[t + off];
                 const long long ix = sx[t + off];
@@ -661,10 +790,11 @@ static unsigned nblocks(size_t n, unsigned bs, unsigned cap)
     return (unsigned)std::max<size_t>(1, std::min<size_t>(b, cap));
 }
 
-int fused_alloc(FusedState &fs, const DevGeom &g, const DevState &d, bool fp32, AllocFn alloc)
+int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
 {
-    fs.fp32 = fp32;
-    fs.esize = fp32 ? 4 : 8;
+    using T = FT;
+    fs.fp32 = sizeof(T) == 4;
+    fs.esize = sizeof(T);
     const size_t N = d.N;
     int rc;
     if ((rc = alloc(9 * N * fs.esize, &fs.Un)) || (rc = alloc(9 * N * fs.esize, &fs.A)) ||
@@ -674,8 +804,8 @@ int fused_alloc(FusedState &fs, const DevGeom &g, const DevState &d, bool fp32, 
     void *p;
     if ((rc = alloc(9 * N * sizeof(double), &p))) return rc;
     fs.u64_scratch = (double *)p;
-    const int TJ = fp32 ? Tile<float>::TJ : Tile<double>::TJ;
-    const int TK = fp32 ? Tile<float>::TK : Tile<double>::TK;
+    const int TJ = Tile<T>::TJ;
+    const int TK = Tile<T>::TK;
     fs.tiles_j = (g.NY - 2 + TJ - 1) / TJ;
     fs.tiles_k = (g.NZ - 2 + TK - 1) / TK;
     const int cols = fs.tiles_j * fs.tiles_k;
@@ -687,6 +817,16 @@ int fused_alloc(FusedState &fs, const DevGeom &g, const DevState &d, bool fp32, 
     fs.dt_blocks = std::max((cols * fs.nseg + 7) / 8 * 8, 1024);
     if ((rc = alloc(2 * (size_t)fs.dt_blocks * sizeof(double), &p))) return rc;
     fs.dtpart = (double *)p;
+    const size_t nik = (size_t)g.NX * g.NZ;
+    if ((rc = alloc((size_t)g.NX * IT * sizeof(double), &p))) return rc;
+    fs.itab = (double *)p;
+    if ((rc = alloc((size_t)g.NZ * KT * sizeof(double), &p))) return rc;
+    fs.ktab = (double *)p;
+    if ((rc = alloc(4 * nik * sizeof(double), &p))) return rc;
+    fs.ikd = (double *)p;
+    if ((rc = alloc(4 * nik * sizeof(T), &fs.ikg))) return rc;
+    hipLaunchKernelGGL(k_pack<T>, dim3(nblocks(nik, 256, 4096)), dim3(256), 0, 0, g, fs.itab, (T *)fs.ikg, fs.ikd, fs.ktab);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return CYLMHD_EHIP;
     return 0;
 }
 
@@ -704,31 +844,28 @@ static FArgs<T> make_args(FusedState &fs, const DevGeom &g, const DevState &d, c
     a.seglen = fs.seglen; a.tiles_j = fs.tiles_j; a.tiles_k = fs.tiles_k; a.ntiles = fs.tiles_j * fs.tiles_k;
     a.Un = (const T *)fs.Un;
     a.shadow = (T *)fs.shadow;
+    a.itab = fs.itab; a.ikd = fs.ikd; a.ktab = fs.ktab; a.ikg = (const T *)fs.ikg;
     return a;
 }
 
-int fused_set_rhofloor(hipStream_t s, FusedState &fs, const DevState &d, double rhocr)
+int set_rhofloor(hipStream_t s, FusedState &fs, const DevState &d, double rhocr)
 {
-    if (fs.fp32) hipLaunchKernelGGL(k_rhofloor<float>, dim3(nblocks(d.N, 256, 8192)), dim3(256), 0, s, d.Uinit, (float *)fs.rhofloor, d.N, rhocr);
-    else hipLaunchKernelGGL(k_rhofloor<double>, dim3(nblocks(d.N, 256, 8192)), dim3(256), 0, s, d.Uinit, (double *)fs.rhofloor, d.N, rhocr);
+    hipLaunchKernelGGL(k_rhofloor<FT>, dim3(nblocks(d.N, 256, 8192)), dim3(256), 0, s, d.Uinit, (FT *)fs.rhofloor, d.N, rhocr);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
-int fused_load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const double *U64)
+int load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const double *U64)
 {
     (void)g;
     const size_t n = 9 * d.N;
-    if (fs.fp32) hipLaunchKernelGGL(k_to_state<float>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, U64, (float *)fs.Un, n);
-    else hipLaunchKernelGGL(k_to_state<double>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, U64, (double *)fs.Un, n);
+    hipLaunchKernelGGL(k_to_state<FT>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, U64, (FT *)fs.Un, n);
     fs.dt_valid = false;
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
-int fused_export(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c, double *V64,
-                 const double *)
+int export_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c, double *V64)
 {
-    if (fs.fp32) hipLaunchKernelGGL(k_export<float>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, c, (const float *)fs.Un, fs.u64_scratch, V64, d.N);
-    else hipLaunchKernelGGL(k_export<double>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, c, (const double *)fs.Un, fs.u64_scratch, V64, d.N);
+    hipLaunchKernelGGL(k_export<FT>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, c, (const FT *)fs.Un, fs.u64_scratch, V64, d.N);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
@@ -747,10 +884,9 @@ static void setdt_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
     hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, fs.dtpart, nb, d.red);
 }
 
-int fused_setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c)
+int setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c)
 {
-    if (fs.fp32) setdt_t<float>(s, fs, g, d, c);
-    else setdt_t<double>(s, fs, g, d, c);
+    setdt_t<FT>(s, fs, g, d, c);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
@@ -775,34 +911,29 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
     const int ne = (g.NY - 2) * (g.NZ - 2);
     hipLaunchKernelGGL((k_edge<T, MC, HLL>), dim3((ne + 255) / 256), dim3(256), 0, s, a, (T *)fs.edge);
     if (ev0) hipEventRecord(ev0, s);
-    hipLaunchKernelGGL((k_fused<T, MC, HLL>), dim3(nb), dim3((Tile<T>::TJ + 1) * Tile<T>::TK), 0, s, a);
+    hipLaunchKernelGGL((k_fused<T, MC, HLL>), dim3(nb), dim3((Tile<T>::TJ + 2) * Tile<T>::TK), 0, s, a);
     if (ev1) hipEventRecord(ev1, s);
     const size_t nghost = 2 * (size_t)g.NY * g.NZ + (size_t)(g.NX - 2) * 2 * g.NZ + (size_t)(g.NX - 2) * (g.NY - 2) * 2;
     hipLaunchKernelGGL(k_halo<T>, dim3(nblocks(nghost, 256, 4096)), dim3(256), 0, s, a);
     if (last) fs.dt_valid = true;
 }
 
-int fused_stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
-                const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1)
+int stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
+          const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1)
 {
-    if (fs.fp32) {
-        if (a.mc) { if (a.hll) stage_t<float, true, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<float, true, false>(s, fs, g, d, c, a, ev0, ev1); }
-        else      { if (a.hll) stage_t<float, false, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<float, false, false>(s, fs, g, d, c, a, ev0, ev1); }
-    } else {
-        if (a.mc) { if (a.hll) stage_t<double, true, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<double, true, false>(s, fs, g, d, c, a, ev0, ev1); }
-        else      { if (a.hll) stage_t<double, false, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<double, false, false>(s, fs, g, d, c, a, ev0, ev1); }
-    }
+    if (a.mc) { if (a.hll) stage_t<FT, true, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<FT, true, false>(s, fs, g, d, c, a, ev0, ev1); }
+    else      { if (a.hll) stage_t<FT, false, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<FT, false, false>(s, fs, g, d, c, a, ev0, ev1); }
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
-int fused_diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d)
+int diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d)
 {
     const size_t M = (size_t)(g.NX - 2) * (g.NY - 2) * (g.NZ - 2);
     const int nb = (int)nblocks(M, 256, (unsigned)(d.nred / 4));
-    if (fs.fp32) hipLaunchKernelGGL(k_diag_t<float>, dim3(nb), dim3(256), 0, s, g, (const float *)fs.Un, d.N, d.red);
-    else hipLaunchKernelGGL(k_diag_t<double>, dim3(nb), dim3(256), 0, s, g, (const double *)fs.Un, d.N, d.red);
+    hipLaunchKernelGGL(k_diag_t<FT>, dim3(nb), dim3(256), 0, s, g, (const FT *)fs.Un, d.N, d.red);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, d.red, nb);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
+}  // namespace FUSED_NS
 }  // namespace cyl

@@ -95,7 +95,14 @@ int cylmhd_params_config(cylmhd_params *p, int config, int nr, int nphi, int nz)
     p->epssp0 = config == 1 ? 0.0 : (config == 5 ? 0.10 : 0.05);
     p->armnum = 2;
     p->pitch = 15.0 * M_PI / 180.0;
-    p->cooling = CYLMHD_COOL_RAYMOND;
+    /* Discs 1-4 carry no cooling: the reference's Raymond fit is in its
+     * galactic units (te0), and in these units its 2 rho^2 Lambda/(1.5 p)
+     * CFL term collapses dt within ~40 steps.  The pressure floor is put at
+     * 10 % of the disc's p/rho = cs^2 = 0.01: the reference's prcr = 0.015
+     * lies above it and would floor every cell (and zero vr, vz) at every
+     * stage. */
+    p->cooling = CYLMHD_COOL_NONE;
+    p->prcr = 1.0e-3;
     p->beta_mode = 0; p->disc_beta = 10.0;
     if (config == 3) { p->beta_mode = 1; p->beta_lo = 0.1; p->beta_hi = 100.0; }
     if (config == 4) { p->beta_mode = 1; p->beta_lo = 1.0; p->beta_hi = 100.0; }

@@ -14,6 +14,9 @@ value     = interior cells x K / wall seconds of K steps (state resident in
 roofline  = the fused stage kernel: algorithmic HBM bytes per launch
             (288 B per cell-update / 3 launches, FP32 state) / its average
             duration from HIP events on the library's stream.
+precision = relative L1 / Linf error per conserved variable of the FP32
+            path against the plain FP64 path after 100 steps on the same
+            dt sequence, with div B and floor hits (precision.py).
 cpu_baseline = the CPU restatement of the reference (oracle, OpenMP on all
             host cores) or the reference itself (oracle/_ref bench build),
             on a bounded sample of the same disc.
@@ -108,6 +111,8 @@ def main():
     ap.add_argument("--grid", type=int, nargs=3, default=[512, 512, 256])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison paths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--precision-steps", type=int, default=100,
+                    help="steps of the FP32-vs-FP64 error report (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); the path would shard as "
@@ -156,6 +161,13 @@ def main():
     if fp64:
         out["fp64"] = {k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms", "steps")} for k, v in fp64.items()}
         out["fp32_over_fp64"] = {k: main_res["value"] / v["value"] for k, v in fp64.items()}
+    if a.precision_steps > 0:
+        import precision
+        try:
+            out["precision"] = precision.compare(cm.params_config(3, *grid), a.precision_steps,
+                                                 cm.PATH_PLAIN64, cm.PATH_FUSED32, log=log)
+        except Exception as e:                     # reported, never fatal to the GPU number
+            out["precision"] = {"error": str(e)}
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cm, np)

@@ -90,9 +90,12 @@ typedef struct cylmhd_params {
 /* Fill *p with the reference's own input.h / constant.h values. */
 int cylmhd_params_default(cylmhd_params *p);
 
-/* Fill *p for one of the benchmark discs:
+/* Fill *p for one of the benchmark discs (r in [0.1,1], z in [-0.5,0.5],
+ * point-mass gravity GM = 1, inertial frame, no cooling unless stated,
+ * pressure floor prcr = 1e-3, i.e. 10 % of the disc's p/rho):
  *   1: 64x64x32, minmod, CPU-sized;     2: 256x256x128, MC, spiral 5%;
  *   3: 512x512x256, MC, beta log-uniform [0.1,100] per annulus;
+ *   4: one disc of the batch (64x64x32 default), minmod, beta [1,100];
  *   5: 768x768x384, MC, power-law cooling, spiral 10%, GLM alpha 0.1.
  * nr/nphi/nz > 0 override the grid size (same disc, other resolution). */
 int cylmhd_params_config(cylmhd_params *p, int config, int nr, int nphi, int nz);

@@ -17,6 +17,17 @@ import cylmhd as cm
 REF_OMEGASP = 10.0e+5 / (1.0 / 4.7058e+15) * (1.0 / 3.08568e+21)
 
 
+# The reference's compile-time floors and its (only) cooling law
+# (input.h:78-80, cooling.h:22-78): what the reference harness is built with.
+REF_FIXED = dict(rhocr=1.0e-4, prcr=0.015, betacr=0.1, cooling=cm.COOL_RAYMOND)
+
+
+def ref_disc_case(config=1, nr=16, nphi=16, nz=10, **over):
+    """A benchmark disc run under the reference's own floors and cooling,
+    for comparisons with the compiled reference."""
+    return disc_case(config, nr, nphi, nz, **{**REF_FIXED, **over})
+
+
 def disc_params(config=1, nr=16, nphi=16, nz=10, **over):
     p = cm.params_config(config, nr, nphi, nz)
     for k, v in over.items():
@@ -37,6 +48,8 @@ def rand_params(nr=16, nphi=8, nz=10, **over):
     p.gravity = cm.GRAV_GALACTIC
     p.epssp0 = 0.0
     p.tausp = 200.0 * 1.0e+6 * 3.1536e+7 / 4.7058e+15
+    for k, v in REF_FIXED.items():
+        setattr(p, k, v)
     for k, v in over.items():
         setattr(p, k, v)
     return p
@@ -101,6 +114,42 @@ def rel_err(a, b):
         num = np.max(np.abs(a[m] - b[m]))
         out.append(num / den if den > 0 else num)
     return np.array(out)
+
+
+def group_scales(b, x, l1=False, interior=True):
+    """Per-component error scale of a conserved state b[9][NX][NY][NZ].
+
+    Components that vanish in the initial disc (rho vr, rho vz, Br, Bz, psi)
+    hold only what one step creates, so an error relative to their own
+    magnitude measures rounding against a near-zero quantity.  Each
+    component is therefore measured against the larger of its own size and
+    the size of the vector it belongs to: rho v (with U2 / r for the phi
+    part) for 1-3, B for 4-6 and for psi (the field it cleans).  Density,
+    angular momentum, B_phi and energy keep their own scale.
+    With l1=True the scales are sums over the (interior) cells, else maxima."""
+    if interior:
+        sl = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+        b = b[sl]
+        x = x[1:-1]
+    mom = np.sqrt(b[1] ** 2 + (b[2] / x[:, None, None]) ** 2 + b[3] ** 2)
+    bmag = np.sqrt(b[4] ** 2 + b[5] ** 2 + b[6] ** 2)
+    red = np.sum if l1 else np.max
+    own = np.array([red(np.abs(b[m])) for m in range(9)])
+    vec = {1: red(mom), 3: red(mom), 4: red(bmag), 6: red(bmag), 8: red(bmag)}
+    return np.array([max(own[m], vec.get(m, 0.0)) for m in range(9)])
+
+
+def rel_err_scaled(a, b, x):
+    """max |a-b| per component over all cells / group_scales (max)."""
+    num = np.array([np.max(np.abs(a[m] - b[m])) for m in range(9)])
+    return num / group_scales(b, x, l1=False, interior=False)
+
+
+def l1_rel_scaled(a, b, x):
+    """sum |a-b| per component over interior cells / group_scales (sum)."""
+    sl = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    num = np.array([np.sum(np.abs(a[sl][m] - b[sl][m])) for m in range(9)])
+    return num / group_scales(b, x, l1=True, interior=True)
 
 
 def l1_rel(a, b, interior=True):

@@ -22,7 +22,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [os.path.join(ROOT, "171026-1-3dhlld-cyli-gal_amd"), os.path.join(ROOT, "tests")]
 
 import cylmhd as cm  # noqa: E402
-from cases import disc_case, rand_case  # noqa: E402
+from cases import rand_case, ref_disc_case  # noqa: E402
 from oracle_py import Oracle, parse_counts, ref_harness, run_ref  # noqa: E402
 
 CASES = {
@@ -39,7 +39,7 @@ def build_inputs(kind, kw):
     if kind == "disc":
         kw = dict(kw)
         cfg = kw.pop("config")
-        return disc_case(cfg, kw.pop("nr"), kw.pop("nphi"), kw.pop("nz"), **kw)
+        return ref_disc_case(cfg, kw.pop("nr"), kw.pop("nphi"), kw.pop("nz"), **kw)
     return rand_case(**kw)
 
 

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import cylmhd as cm
-from cases import disc_case, rand_case
+from cases import rand_case, ref_disc_case
 from oracle_py import Oracle, parse_counts, parse_dt_mesh, ref_harness, run_ref
 
 SCHEMES = {"mm_hlld": (cm.LIMITER_MINMOD, cm.SOLVER_HLLD), "mc_hlld": (cm.LIMITER_MC, cm.SOLVER_HLLD),
@@ -58,7 +58,7 @@ def compare(name, p, Ui, Vi, Uc, Vc, nsteps, galactic):
 @pytest.mark.parametrize("scheme", list(SCHEMES))
 def test_disc_matches_reference(scheme, rk):
     lim, sol = SCHEMES[scheme]
-    p, Ui, Vi, Uc, Vc = disc_case(1, 16, 16, 10, limiter=lim, solver=sol, rk=rk, epssp0=0.0)
+    p, Ui, Vi, Uc, Vc = ref_disc_case(1, 16, 16, 10, limiter=lim, solver=sol, rk=rk, epssp0=0.0)
     compare(f"disc16_{scheme}", p, Ui, Vi, Uc, Vc, 4, galactic=False)
 
 
@@ -99,7 +99,7 @@ def test_geometry_and_tables_match_reference():
     name = "disc16_mm_hlld"
     if ref_harness(name) is None:
         pytest.skip("reference harness not built")
-    p, Ui, Vi, Uc, Vc = disc_case(1, 16, 16, 10, epssp0=0.0)
+    p, Ui, Vi, Uc, Vc = ref_disc_case(1, 16, 16, 10, epssp0=0.0)
     o = Oracle(p)
     gx, gz = o.gravity_axi()
     r, _ = run_ref(name, Vi, Vc, gx, gz, 0, 3)
@@ -139,7 +139,7 @@ def test_binary_dump_is_byte_identical(tmp_path):
     name = "disc16_mm_hlld"
     if ref_harness(name) is None:
         pytest.skip("reference harness not built")
-    p, Ui, Vi, Uc, Vc = disc_case(1, 16, 16, 10, epssp0=0.0)
+    p, Ui, Vi, Uc, Vc = ref_disc_case(1, 16, 16, 10, epssp0=0.0)
     o, U, V, hist = run_oracle(p, Ui, Vi, Uc, Vc, 2)
     gx, gz = o.gravity_axi()
     refdir = tmp_path / "ref"
