@@ -312,22 +312,24 @@ __global__ __launch_bounds__((Tile<T>::TJ + 2) * Tile<T>::TK, Tile<T>::WAVES) vo
             for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
         }
     };
+    int hj = 0, hk = 0;
+    if (t < 2 * TK) { hj = -2 + t / TK; hk = t % TK; }
+    else if (t < 4 * TK) { hj = TJ + (t - 2 * TK) / TK; hk = (t - 2 * TK) % TK; }
+    else if (t < 4 * TK + 2 * TJ) { const int u = t - 4 * TK; hj = u / 2; hk = -2 + u % 2; }
+    else { const int u = t - 4 * TK - 2 * TJ; hj = u / 2; hk = TK + u % 2; }
+    const size_t hcol = (size_t)jmap(g, j0 + hj) * g.NZ + kmap(g, k0 + hk);
+    const int hoff = (hj + 2) * PK + hk + 2;
     auto load_halo = [&](int p) {
         if (t < NHALO) {
-            int jj, kk;
-            if (t < 2 * TK) { jj = -2 + t / TK; kk = t % TK; }
-            else if (t < 4 * TK) { jj = TJ + (t - 2 * TK) / TK; kk = (t - 2 * TK) % TK; }
-            else if (t < 4 * TK + 2 * TJ) { const int u = t - 4 * TK; jj = u / 2; kk = -2 + u % 2; }
-            else { const int u = t - 4 * TK - 2 * TJ; jj = u / 2; kk = TK + u % 2; }
             T v[9];
-            cell_V(a, C3(p, jmap(g, j0 + jj), kmap(g, k0 + kk)), p, v);
-            const int off = (p % 3) * SLOT + (jj + 2) * PK + kk + 2;
+            cell_V(a, (size_t)p * g.NY * g.NZ + hcol, p, v);
+            const int off = (p % 3) * SLOT + hoff;
 #pragma unroll
             for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
         }
     };
 
-    T up_prev[9], fx_prev[9], acc[9];
+    T up_prev[9], fx_prev[9], fxn[9];
     int f0 = 0, f1 = 0, f2 = 0;
     double best = 0.0;
     long long bix = LLONG_MAX;
@@ -376,7 +378,6 @@ __global__ __launch_bounds__((Tile<T>::TJ + 2) * Tile<T>::TK, Tile<T>::WAVES) vo
         const double *it = a.itab + (size_t)i * IT;
         /* ---- x: flux at i+1/2, divergence term of the r faces ------------ */
         if (is_cell) {
-            T fxn[9];
             if (i == g.NX - 2) {
                 const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
 #pragma unroll
@@ -389,10 +390,6 @@ __global__ __launch_bounds__((Tile<T>::TJ + 2) * Tile<T>::TK, Tile<T>::WAVES) vo
 #pragma unroll
                 for (int m = 0; m < 9; m++) up_prev[m] = up[m];
             }
-            const size_t ik = (size_t)i * g.NZ + kmap(g, gk);
-            const T g1 = a.ikg[4 * ik + 0], g2 = a.ikg[4 * ik + 1];
-#pragma unroll
-            for (int m = 0; m < 9; m++) { acc[m] = g2 * fxn[m] - g1 * fx_prev[m]; fx_prev[m] = fxn[m]; }
         }
         /* ---- limiter states, once per cell and direction ---------------- */
         {
@@ -467,6 +464,12 @@ __global__ __launch_bounds__((Tile<T>::TJ + 2) * Tile<T>::TK, Tile<T>::WAVES) vo
         /* ---- divergence, sources, stage update, absorb, Check, store ----- */
         if (valid) {
             const size_t ik = (size_t)i * g.NZ + gk;
+            T acc[9];
+            {
+                const T g1 = a.ikg[4 * ik + 0], g2 = a.ikg[4 * ik + 1];
+#pragma unroll
+                for (int m = 0; m < 9; m++) { acc[m] = g2 * fxn[m] - g1 * fx_prev[m]; fx_prev[m] = fxn[m]; }
+            }
             {
                 const T g3 = a.ikg[4 * ik + 2], g4 = g3;
                 const int qm = (cj + 1) * TK + ck, qp = qm + TK;
@@ -495,14 +498,19 @@ __global__ __launch_bounds__((Tile<T>::TJ + 2) * Tile<T>::TK, Tile<T>::WAVES) vo
                 for (int m = 0; m < 9; m++) un[m] = a.Un[m * N + cc];
                 un[8] = T(c.psicoef) * un[8];
             }
-            const T dt = T(c.dt), oneo3 = T(1.0) / T(3.0);
+            /* The RK combination is formed in FP64 and rounded once: with FP32
+             * state the stage-2/3 forms would otherwise round four or five
+             * times per update (for FP64 state this is the same arithmetic). */
+            const double dt = c.dt, oneo3 = 1.0 / 3.0;
 #pragma unroll
             for (int m = 0; m < 9; m++) {
-                const T div = acc[m];
-                if (kind == 1 || kind == 4) u[m] = u[m] - dt * div + dt * S[m];
-                else if (kind == 2) u[m] = T(0.25) * (T(3.0) * un[m] + u[m] - dt * div + dt * S[m]);
-                else if (kind == 3) u[m] = oneo3 * (un[m] + T(2.0) * (u[m] - dt * div + dt * S[m]));
-                else u[m] = T(0.5) * (un[m] + u[m] - dt * div + dt * S[m]);
+                const double div = acc[m], um = u[m], unm = un[m], sm = S[m];
+                double r;
+                if (kind == 1 || kind == 4) r = um - dt * div + dt * sm;
+                else if (kind == 2) r = 0.25 * (3.0 * unm + um - dt * div + dt * sm);
+                else if (kind == 3) r = oneo3 * (unm + 2.0 * (um - dt * div + dt * sm));
+                else r = 0.5 * (unm + um - dt * div + dt * sm);
+                u[m] = T(r);
             }
             if (!a.rk3 && gk == 1) u[3] = T(0.0);       /* ZLeftVzAntisymBSym, U3[k=1] */
             if (i == 1 || i == g.NX - 2 || gj == 1 || gj == g.NY - 2 || gk <= 2 || gk == g.NZ - 2) {
@@ -537,6 +545,9 @@ __global__ __launch_bounds__((Tile<T>::TJ + 2) * Tile<T>::TK, Tile<T>::WAVES) vo
                 const long long ix = (long long)cc;
                 if ((tt > best || (tt == best && ix < bix)) && tt > 0.0) { best = tt; bix = ix; }
             }
+        } else if (is_cell) {
+#pragma unroll
+            for (int m = 0; m < 9; m++) fx_prev[m] = fxn[m];
         }
     }
     /* ---- floor counters: one atomic per wave ---------------------------- */

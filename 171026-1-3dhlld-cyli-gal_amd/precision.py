@@ -4,7 +4,8 @@ Both paths advance the same initial state with the SAME dt/ch sequence
 (taken from the FP64 run, as main.c:324-348 would compute it), so the
 difference measured is that of the arithmetic, not of the time steps.
 Reported per conserved variable: relative L1 and L-infinity error over the
-interior, with the components that vanish in the initial disc (rho vr,
+interior, and over the cells outside the absorbing layers (the evolved
+solution), with the components that vanish in the initial disc (rho vr,
 rho vz, Br, Bz, psi) measured against the vector they belong to (rho v,
 B); plus L1 div B of both runs and the floor hits of the FP32 run.
 """
@@ -25,13 +26,26 @@ def _scales(b, x, red):
     return np.array([max(own[m], vec.get(m, 0.0)) for m in range(9)])
 
 
-def errors(a, b, x):
-    """(L1, Linf) relative error per component of a against b, interior."""
-    sl = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+def errors(a, b, x, ilo=1, ihi=None):
+    """(L1, Linf) relative error per component of a against b over the
+    interior cells with ilo <= i <= ihi (default: all interior cells)."""
+    ihi = a.shape[1] - 2 if ihi is None else ihi
+    sl = (slice(None), slice(ilo, ihi + 1), slice(1, -1), slice(1, -1))
     d = np.abs(a[sl] - b[sl])
-    l1 = np.array([d[m].sum() for m in range(9)]) / _scales(b, x, np.sum)
-    linf = np.array([d[m].max() for m in range(9)]) / _scales(b, x, np.max)
+    bb = b[:, ilo - 1:ihi + 2]           # _scales strips one ghost layer
+    xx = x[ilo - 1:ihi + 2]
+    l1 = np.array([d[m].sum() for m in range(9)]) / _scales(bb, xx, np.sum)
+    linf = np.array([d[m].max() for m in range(9)]) / _scales(bb, xx, np.max)
     return l1, linf
+
+
+def physical_range(p):
+    """i-range outside the absorbing layers: the central region i <= irin,
+    irin = (int)(rin/dx0) (input.h:72, centralboundary.h:27-52), and the
+    outer 11 r-cells (xboundary.h:322-346) are relaxed towards the initial
+    state every step, so they are not part of the evolved solution."""
+    irin = int(p.rin / p.dx0)
+    return irin + 1, p.ix + 2 - 12
 
 
 def compare(p, steps, ref_path=cm.PATH_PLAIN64, test_path=cm.PATH_FUSED32, log=None):
@@ -69,6 +83,8 @@ def compare(p, steps, ref_path=cm.PATH_PLAIN64, test_path=cm.PATH_FUSED32, log=N
         d32 = g.diagnostics()
         U32, _ = g.download()
     l1, linf = errors(U32, U64, x)
+    ilo, ihi = physical_range(p)
+    pl1, plinf = errors(U32, U64, x, ilo, ihi)
     del U32, U64
     return dict(
         steps=steps, t=t, grid=[p.ix, p.jy, p.kz],
@@ -76,6 +92,10 @@ def compare(p, steps, ref_path=cm.PATH_PLAIN64, test_path=cm.PATH_FUSED32, log=N
         l1={k: float(v) for k, v in zip(NAMES, l1)},
         linf={k: float(v) for k, v in zip(NAMES, linf)},
         l1_max=float(l1.max()),
+        physical={"i_range": [ilo, ihi],
+                  "l1": {k: float(v) for k, v in zip(NAMES, pl1)},
+                  "linf": {k: float(v) for k, v in zip(NAMES, plinf)},
+                  "l1_max": float(pl1.max())},
         divb_l1={"ref": d64["divb_l1"], "test": d32["divb_l1"]},
         mass_rel=abs(d32["mass"] - d64["mass"]) / abs(d64["mass"]),
         energy_rel=abs(d32["energy"] - d64["energy"]) / abs(d64["energy"]),

@@ -9,13 +9,16 @@
  * #defines, input.h:45-72); everything else is the reference's code.
  *
  * Deliberately NOT compiled: spiralgravity.h (it calls cot(), which no libm
- * defines, spiralgravity.h:39), model.h and perturbation.h (initial states
- * come from the input file).  Gravity is Grxaxi.. (from the file or from the
- * reference's InitialGravity) with the spiral part off.
+ * defines, spiralgravity.h:39).  Gravity is Grxaxi.. (from the file, from
+ * the reference's InitialGravity or from GalacticDisc3) with the spiral part
+ * off.  Initial states come from the input file, or (flags bit 2) from the
+ * reference's GalacticDisc3 (model.h, which needs perturbation.h).
  *
  *   usage: ref_harness IN OUT NSTEPS RK [DUMPDIR]
  *   IN : int64 NX,NY,NZ,flags; V_init[9N]; V_cur[9N]; grx[NX*NZ]; grz[NX*NZ]
  *        flags bit0: use InitialGravity() instead of the tables
+ *              bit1: U_init[9N], U_cur[9N] follow (conserved states given)
+ *              bit2: initial state and gravity from GalacticDisc3()
  *   OUT: int64 NX,NY,NZ,nsteps; dt[n]; ch[n]; U[9N]; V[9N]; F,G,H[9*NF] of the
  *        last stage; x,xm[NX]; y[NY]; z[NZ]; minminlength; g1,g2,g3,g5,
  *        dvolume,minlength [NX*NZ] (j = 1); grxaxi,grzaxi[NX*NZ];
@@ -99,6 +102,8 @@
 #include "secondstep.h"
 #include "thirdstep.h"
 #include "check.h"
+#include "perturbation.h"
+#include "model.h"
 #include "output.h"
 
 #define NCELL ((size_t)ixmax * jymax * kzmax)
@@ -142,15 +147,24 @@ int main(int argc, char **argv)
     fclose(fi);
 
     CylindricalGrid();
-    /* initial state -> Uinitial/Vinitial/rhofloor (saveinitial.h) */
-    if (hdr[3] & 2) memcpy(U, Uini, sizeof U);
-    else VtoU();
-    SaveInitial();
-    /* current state */
-    memcpy(V, Vcur, sizeof V);
-    if (hdr[3] & 2) memcpy(U, Ucur, sizeof U);
-    else VtoU();
-    if (hdr[3] & 1) {
+    if (hdr[3] & 4) {
+        /* the reference's own initial state: GalacticDisc3 (model.h:149-327)
+         * sets V, U (with its z boundaries), the axisymmetric gravity and
+         * SaveInitial; the input states are ignored */
+        GalacticDisc3();
+    } else {
+        /* initial state -> Uinitial/Vinitial/rhofloor (saveinitial.h) */
+        if (hdr[3] & 2) memcpy(U, Uini, sizeof U);
+        else VtoU();
+        SaveInitial();
+        /* current state */
+        memcpy(V, Vcur, sizeof V);
+        if (hdr[3] & 2) memcpy(U, Ucur, sizeof U);
+        else VtoU();
+    }
+    if (hdr[3] & 4) {
+        /* gravity as GalacticDisc3 left it */
+    } else if (hdr[3] & 1) {
         InitialGravity();
     } else {
         for (i = 0; i < ixmax; i++)

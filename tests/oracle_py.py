@@ -189,10 +189,12 @@ def ref_harness(name):
     return path if os.path.isfile(path) else None
 
 
-def run_ref(name, V_init, V_cur, grx, grz, nsteps, rk, galactic=False, dumpdir=None, U_init=None, U_cur=None):
+def run_ref(name, V_init, V_cur, grx, grz, nsteps, rk, galactic=False, dumpdir=None, U_init=None, U_cur=None,
+            model=False):
     """Run one compiled reference harness; returns (dict of arrays, stdout).
     With U_init/U_cur the harness takes the conserved states as given
-    instead of rebuilding them with VtoU."""
+    instead of rebuilding them with VtoU; with model=True it starts from the
+    reference's own GalacticDisc3 (the input states only set the sizes)."""
     exe = ref_harness(name)
     if exe is None:
         raise FileNotFoundError(name)
@@ -202,7 +204,7 @@ def run_ref(name, V_init, V_cur, grx, grz, nsteps, rk, galactic=False, dumpdir=N
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         with open(fin, "wb") as f:
-            flags = (1 if galactic else 0) | (2 if U_init is not None else 0)
+            flags = (1 if galactic else 0) | (2 if U_init is not None else 0) | (4 if model else 0)
             np.array([NX, NY, NZ, flags], dtype=np.int64).tofile(f)
             np.ascontiguousarray(V_init, dtype=np.float64).tofile(f)
             np.ascontiguousarray(V_cur, dtype=np.float64).tofile(f)

@@ -156,3 +156,29 @@ def test_binary_dump_is_byte_identical(tmp_path):
     # RestartRead (restartread.h:9-143) reads the nine primitive files back
     V2 = cm.restart_read(str(mine), V.shape[1:])
     np.testing.assert_array_equal(V2, V)
+
+
+def test_galactic_disc3_matches_reference_model():
+    """GalacticDisc3 (model.h:149-327) compiled from the reference vs the
+    host restatement (cylmhd_model_galactic_disc3): primitives bit for bit,
+    U = VtoU(V) with the k = 0, 1 planes of its z boundary pair
+    (ZLeftVzAntisymBSymBoundary then ZLeftFreeBoundary), and its gravity
+    equal to InitialGravity's (the oracle's tables)."""
+    name = "rand16_mm_hlld"
+    if ref_harness(name) is None:
+        pytest.skip("reference harness not built")
+    p, Ui, Vi, Uc, Vc = rand_case()
+    V = cm.model_galactic_disc3(p)
+    r, _ = run_ref(name, Vi, Vc, np.zeros(Vi.shape[1::2]), np.zeros(Vi.shape[1::2]), 0, 3, model=True)
+    np.testing.assert_array_equal(r["V"], V)
+    U = cm.vtou(p, V)
+    np.testing.assert_array_equal(r["U"][..., 2:], U[..., 2:])
+    np.testing.assert_array_equal(r["U"][3, :, :, 1], 0.0)
+    keep = [m for m in range(9) if m != 3]
+    np.testing.assert_array_equal(r["U"][keep][..., 1], U[keep][..., 1])
+    np.testing.assert_array_equal(r["U"][..., 0], r["U"][..., 1])
+    o = Oracle(p)
+    gx, gz = o.gravity_axi()
+    o.close()
+    np.testing.assert_array_equal(r["grx"], gx)
+    np.testing.assert_array_equal(r["grz"], gz)
