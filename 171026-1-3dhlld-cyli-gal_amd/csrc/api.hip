@@ -13,6 +13,7 @@
 #include <cstring>
 #include <vector>
 
+#include "batch.h"
 #include "fused.h"
 #include "paths.h"
 
@@ -222,6 +223,33 @@ int alloc_state(cylmhd_ctx *c)
 }
 
 }  // namespace
+
+namespace cyl {
+
+int disc_view(cylmhd_ctx *c, DiscView *v)
+{
+    if (!c || !v || c->p.path != CYLMHD_PATH_FUSED32) return CYLMHD_EINVAL;
+    v->p = &c->p;
+    v->dg = c->dg;
+    v->ds = c->ds;
+    v->fs = &c->fs;
+    v->sincos = c->d_sincos;
+    v->logxs = c->d_logxs;
+    v->minminlength = c->g.minminlength;
+    v->stream = c->stream;
+    return CYLMHD_OK;
+}
+
+Consts disc_consts(const cylmhd_ctx *c, double dt, double ch) { return make_consts(c, dt, ch); }
+
+void disc_invalidate_dt(cylmhd_ctx *c)
+{
+    if (c) c->fs.dt_valid = false;
+}
+
+void set_errmsg(const char *msg) { snprintf(g_errmsg, sizeof g_errmsg, "%s", msg); }
+
+}  // namespace cyl
 
 extern "C" {
 

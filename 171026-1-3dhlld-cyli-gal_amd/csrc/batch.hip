@@ -1,0 +1,618 @@
+/*
+ * batch.hip -- many independent discs advanced together (FP32 state).
+ *
+ * Every disc keeps its own context (tables, state, Uinitial, counters), so
+ * set-up, dumps and restarts go through the single-disc ABI.  The batch adds:
+ *   - a work list of (disc, tile, r-segment) items over all discs, processed
+ *     by ONE launch per RK stage whose blocks loop over it (XCD-aware order
+ *     within each round of gridDim items);
+ *   - per-disc Setdt on the device: the last stage leaves one partial per
+ *     item; k_dt_batch reduces each disc's range, forms dt = cfl/max,
+ *     ch = minminlength * max, psi damping and the spiral amplitude at the
+ *     disc's own time, and writes them into that disc's stage arguments;
+ *     a disc whose dt falls below dtmin (or is not finite) stops, as main.c
+ *     would stop it (main.c:331-334), while the others go on;
+ *   - so a whole step needs nothing from the host: it is captured once into
+ *     a hipGraph and replayed;
+ *   - an autotuner over the tile shape, the r-segment length and the grid of
+ *     the stage launch, timed with HIP events on the batch stream.
+ */
+#define FUSED_NS b32
+#define FUSED_FT float
+#include "fused_tile.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "batch.h"
+
+namespace cyl {
+namespace b32 {
+
+/* per-disc step state, device resident */
+struct BDisc {
+    double t, dt, ch;
+    int alive, steps;
+    double minminlength, cfl, dtmin, cr, epssp0, tausp, domega;
+    const double *logxs;
+    double *sincos;
+    float *edge;
+    int item_lo, item_hi;
+};
+
+__device__ __forceinline__ int find_disc(const int *__restrict__ off, int n, int b)
+{
+    int lo = 0, hi = n;                       /* off[lo] <= b < off[hi] */
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (off[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <bool MC, bool HLL, int S>
+__global__ __launch_bounds__((tile_threads<float, S>()), (tile_waves<float, S>())) void k_fused_batch(
+    const FArgs<float> *__restrict__ descs, const int4 *__restrict__ items, int nitems, const BDisc *__restrict__ bd)
+{
+    const int b = remap_block(blockIdx.x, gridDim.x);
+    for (int it = b; it < nitems; it += gridDim.x) {
+        const int4 w = items[it];
+        if (!bd[w.x].alive) continue;
+        fused_tile<float, MC, HLL, S>(descs[w.x], w.y, w.z, it);
+    }
+}
+
+template <bool MC, bool HLL>
+__global__ __launch_bounds__(256) void k_edge_batch(const FArgs<float> *__restrict__ descs, const int *__restrict__ off,
+                                                    int n, const BDisc *__restrict__ bd)
+{
+    const int d = find_disc(off, n, blockIdx.x);
+    if (!bd[d].alive) return;
+    edge_cell<float, MC, HLL>(descs[d], bd[d].edge, (blockIdx.x - off[d]) * 256 + threadIdx.x);
+}
+
+__global__ __launch_bounds__(256) void k_halo_batch(const FArgs<float> *__restrict__ descs, const int *__restrict__ off,
+                                                    int n, const BDisc *__restrict__ bd)
+{
+    const int d = find_disc(off, n, blockIdx.x);
+    if (!bd[d].alive) return;
+    const size_t nb = off[d + 1] - off[d];
+    halo_range<float>(descs[d], (size_t)(blockIdx.x - off[d]) * 256 + threadIdx.x, nb * 256);
+}
+
+/* Setdt partials of a stored state (before the first step): the blocks of
+ * disc d are its work-list slots item_lo..item_hi-1 */
+__global__ __launch_bounds__(256) void k_setdt_batch(const FArgs<float> *__restrict__ descs, const BDisc *__restrict__ bd,
+                                                     const int *__restrict__ off, int n, double *__restrict__ dtpart)
+{
+    const int d = find_disc(off, n, blockIdx.x);
+    const FArgs<float> &a = descs[d];
+    const DevGeom &g = a.g;
+    const size_t ni = (size_t)(g.NX - 12), nj = (size_t)(g.NY - 2), nk = (size_t)(g.NZ - 2);
+    const size_t M = ni * nj * nk;
+    const size_t nb = off[d + 1] - off[d];
+    double best = 0.0;
+    long long bix = LLONG_MAX;
+    for (size_t q = (blockIdx.x - off[d]) * (size_t)256 + threadIdx.x; q < M; q += nb * 256) {
+        const int i = 1 + (int)(q / (nj * nk)), j = 1 + (int)((q / nk) % nj), k = 1 + (int)(q % nk);
+        const size_t cc = C3(i, j, k), cs = SI(i, j, k);
+        float u[9], v[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) u[m] = a.Un[m * a.NS + cs];
+        Prec<float>::utov(a.c, u, g.oneox[i], v);
+        const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
+        const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
+        const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
+        const double t1 = sqrt(v2 + (a.c.gm * V7 + b2) / V0) / g.minlength[(size_t)i * g.NZ + k];
+        const double tc = (2.0 * ((V0 * V0 * cooling(a.c, V0, V7)) / (1.5 * V7)));
+        const double tt = rmax(t1, tc);
+        if ((tt > best || (tt == best && (long long)cc < bix)) && tt > 0.0) { best = tt; bix = (long long)cc; }
+    }
+    __shared__ double sv[256];
+    __shared__ long long sx[256];
+    const int t = threadIdx.x;
+    sv[t] = best; sx[t] = bix;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o && (sv[t + o] > sv[t] || (sv[t + o] == sv[t] && sx[t + o] < sx[t]))) { sv[t] = sv[t + o]; sx[t] = sx[t + o]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const int slot = bd[d].item_lo + (int)(blockIdx.x - off[d]);
+        dtpart[2 * slot] = sv[0];
+        dtpart[2 * slot + 1] = (double)sx[0];
+    }
+}
+
+/* per-disc Setdt (setdt.h:24-94 reduced from the partials), TVDRK's psi
+ * damping (tvdrungekutta.h:22) and the spiral amplitude at the disc's time
+ * (spiralgravity.h:41-42), written into the disc's stage arguments */
+__global__ __launch_bounds__(256) void k_dt_batch(FArgs<float> *__restrict__ descs, int nst, int n, BDisc *__restrict__ bd,
+                                                  const double *__restrict__ dtpart)
+{
+    const int d = blockIdx.x;
+    BDisc &D = bd[d];
+    if (!D.alive) return;
+    const int t = threadIdx.x;
+    double best = 0.0;
+    long long bix = LLONG_MAX;
+    for (int q = D.item_lo + t; q < D.item_hi; q += 256) {
+        const double v = dtpart[2 * q];
+        const long long ix = dtpart[2 * q + 1] >= 9.0e18 ? LLONG_MAX : (long long)dtpart[2 * q + 1];
+        if ((v > best || (v == best && ix < bix)) && v > 0.0) { best = v; bix = ix; }
+    }
+    __shared__ double sv[256];
+    __shared__ long long sx[256];
+    sv[t] = best; sx[t] = bix;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o && (sv[t + o] > sv[t] || (sv[t + o] == sv[t] && sx[t + o] < sx[t]))) { sv[t] = sv[t + o]; sx[t] = sx[t + o]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double dtmax = sv[0];
+        const double dt = D.cfl / dtmax, ch = D.minminlength * dtmax;
+        if (!(dt >= D.dtmin) || isinf(dt)) {
+            D.alive = 0;
+            return;
+        }
+        D.dt = dt;
+        D.ch = ch;
+        const double psicoef = exp(-dt * ch * (1.0 / D.cr) * 0.5);
+        const double sw = (D.t - D.tausp) / fabs(D.t - D.tausp);
+        const double epssp = 0.5 * D.epssp0 * (D.t / D.tausp * (1.0 - sw) + (1.0 + sw));
+        for (int s = 0; s < nst; s++) {
+            FArgs<float> &a = descs[(size_t)s * n + d];
+            a.c.dt = dt;
+            a.c.ch = ch;
+            a.c.psicoef = psicoef;
+            a.g.epssp = epssp;
+        }
+    }
+}
+
+/* sin/cos of the arm phase at each disc's time (spiralgravity.h:63-65) */
+__global__ __launch_bounds__(256) void k_sincos_batch(const FArgs<float> *__restrict__ descs, const int *__restrict__ off,
+                                                      int n, const BDisc *__restrict__ bd)
+{
+    const int d = find_disc(off, n, blockIdx.x);
+    const BDisc &D = bd[d];
+    if (!D.alive) return;
+    const DevGeom &g = descs[d].g;
+    const size_t tot = (size_t)g.NX * g.NY, nb = off[d + 1] - off[d];
+    for (size_t q = (blockIdx.x - off[d]) * (size_t)256 + threadIdx.x; q < tot; q += nb * 256) {
+        const int i = (int)(q / g.NY), j = (int)(q % g.NY);
+        const double phase = g.armnum * (g.y[j] - D.domega * D.t + g.cotpitch * D.logxs[i]);
+        double sp, cp;
+        sincos(phase, &sp, &cp);
+        D.sincos[q] = sp;
+        D.sincos[tot + q] = cp;
+    }
+}
+
+__global__ void k_time_batch(BDisc *__restrict__ bd, int n)
+{
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < n && bd[d].alive) {
+        bd[d].t += bd[d].dt;
+        bd[d].steps += 1;
+    }
+}
+
+}  // namespace b32
+}  // namespace cyl
+
+using namespace cyl;
+using namespace cyl::b32;
+
+namespace {
+
+#define BCHK(call)                                                                   \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess) {                                                      \
+            char m_[400];                                                            \
+            snprintf(m_, sizeof m_, "%s: %s", #call, hipGetErrorString(e_));         \
+            set_errmsg(m_);                                                          \
+            return CYLMHD_EHIP;                                                      \
+        }                                                                            \
+    } while (0)
+
+unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+struct cylmhd_batch {
+    int n = 0;
+    std::vector<cylmhd_ctx *> disc;
+    std::vector<DiscView> view;
+    hipStream_t stream = nullptr;
+    bool mc = false, hll = false, rk3 = true;
+    int nst = 3;
+    /* tuned launch shape */
+    int shape = 0, seglen = 0, grid = 1024;
+    /* device side */
+    FArgs<float> *d_descs = nullptr;
+    int4 *d_items = nullptr;
+    BDisc *d_bd = nullptr;
+    int *d_eoff = nullptr, *d_hoff = nullptr, *d_soff = nullptr, *d_ioff = nullptr;
+    double *d_dtpart = nullptr;
+    int nitems = 0, eblocks = 0, hblocks = 0, sblocks = 0;
+    std::vector<int> ioff;
+    std::vector<void *> allocs;
+    float *d_backup = nullptr;
+    size_t backup_elems = 0;
+    bool built = false, primed = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+int balloc(cylmhd_batch *b, void **p, size_t bytes)
+{
+    hipError_t e = hipMalloc(p, bytes + 256);
+    if (e != hipSuccess) {
+        set_errmsg("hipMalloc failed for the batch work list");
+        return CYLMHD_ENOMEM;
+    }
+    b->allocs.push_back(*p);
+    return CYLMHD_OK;
+}
+
+void free_graph(cylmhd_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void free_lists(cylmhd_batch *b)
+{
+    free_graph(b);
+    for (void *p : b->allocs) hipFree(p);
+    b->allocs.clear();
+    b->d_descs = nullptr; b->d_items = nullptr; b->d_bd = nullptr; b->d_dtpart = nullptr;
+    b->d_eoff = b->d_hoff = b->d_soff = b->d_ioff = nullptr;
+    b->built = false;
+}
+
+/* tiles and r-segments of one disc for the chosen shape */
+void disc_tiling(const DiscView &v, int shape, int seglen, int &tiles_j, int &tiles_k, int &seg, int &nseg)
+{
+    const int TJ = shape ? Tile<float, 1>::TJ : Tile<float, 0>::TJ;
+    const int TK = shape ? Tile<float, 1>::TK : Tile<float, 0>::TK;
+    tiles_j = (v.dg.NY - 2 + TJ - 1) / TJ;
+    tiles_k = (v.dg.NZ - 2 + TK - 1) / TK;
+    const int planes = v.dg.NX - 2;
+    seg = seglen > 0 ? std::min(seglen, planes) : planes;
+    nseg = (planes + seg - 1) / seg;
+}
+
+/* work list, stage arguments and per-disc state for the current shape */
+int build(cylmhd_batch *b, bool keep_time)
+{
+    std::vector<BDisc> old;
+    if (keep_time && b->d_bd) {
+        old.resize(b->n);
+        BCHK(hipMemcpy(old.data(), b->d_bd, sizeof(BDisc) * b->n, hipMemcpyDeviceToHost));
+    }
+    free_lists(b);
+    const int n = b->n, nst = b->nst;
+    std::vector<int4> items;
+    std::vector<FArgs<float>> descs((size_t)nst * n);
+    std::vector<BDisc> bd(n);
+    std::vector<int> eoff(n + 1, 0), hoff(n + 1, 0), soff(n + 1, 0);
+    b->ioff.assign(n + 1, 0);
+    for (int d = 0; d < n; d++) {
+        const DiscView &v = b->view[d];
+        int tj, tk, seg, nseg;
+        disc_tiling(v, b->shape, b->seglen, tj, tk, seg, nseg);
+        b->ioff[d] = (int)items.size();
+        for (int sgm = 0; sgm < nseg; sgm++)
+            for (int tile = 0; tile < tj * tk; tile++) items.push_back(make_int4(d, tile, sgm, 0));
+        const Consts c = disc_consts(b->disc[d], 0.0, 0.0);
+        for (int s = 0; s < nst; s++) {
+            FArgs<float> a{};
+            a.g = v.dg;
+            a.c = c;
+            a.N = v.ds.N;
+            a.NS = v.fs->NS; a.NZP = v.fs->NZP;
+            a.rhofloor = (const float *)v.fs->rhofloor;
+            a.Uinit = v.ds.Uinit;
+            a.Vinit_z = v.ds.Vinit_z;
+            a.edge = (const float *)v.fs->edge;
+            a.dtpart = nullptr;            /* set below */
+            a.seglen = seg; a.tiles_j = tj; a.tiles_k = tk; a.ntiles = tj * tk;
+            a.nblocks = tj * tk * nseg;
+            a.Un = (const float *)v.fs->Un;
+            a.shadow = (float *)v.fs->shadow;
+            a.itab = v.fs->itab; a.ikd = v.fs->ikd; a.ktab = v.fs->ktab; a.ikg = (const float *)v.fs->ikg;
+            const int stage = s + 1;
+            const bool last = b->rk3 ? stage == 3 : stage == 2;
+            a.kind = b->rk3 ? stage : 3 + stage;
+            a.last = last;
+            a.rk3 = b->rk3;
+            a.want_dt = last;
+            a.counts = v.ds.counts + 3 * s;
+            a.Uin = stage == 1 ? (const float *)v.fs->Un : (stage == 2 ? (const float *)v.fs->A : (const float *)v.fs->B);
+            a.Uout = last ? (float *)v.fs->Un : (stage == 1 ? (float *)v.fs->A : (float *)v.fs->B);
+            descs[(size_t)s * n + d] = a;
+        }
+        BDisc &D = bd[d];
+        D.t = keep_time ? old[d].t : 0.0;
+        D.dt = keep_time ? old[d].dt : 0.0;
+        D.ch = 0.0;
+        D.alive = keep_time ? old[d].alive : 1;
+        D.steps = keep_time ? old[d].steps : 0;
+        D.minminlength = v.minminlength;
+        D.cfl = v.p->cfl;
+        D.dtmin = v.p->dtmin;
+        D.cr = v.p->cr;
+        D.epssp0 = v.p->epssp0;
+        D.tausp = v.p->tausp;
+        D.domega = v.p->omegapattern - v.p->omegasp;
+        D.logxs = v.logxs;
+        D.sincos = v.sincos;
+        D.edge = (float *)v.fs->edge;
+        eoff[d + 1] = eoff[d] + (int)cdiv((size_t)(v.dg.NY - 2) * (v.dg.NZ - 2), 256);
+        const size_t nh = 2 * (size_t)v.dg.NY * v.dg.NZ + (size_t)(v.dg.NX - 2) * 2 * v.dg.NZ
+                          + (size_t)(v.dg.NX - 2) * (v.dg.NY - 2) * 2;
+        hoff[d + 1] = hoff[d] + (int)std::min<size_t>(cdiv(nh, 256), 512);
+        soff[d + 1] = soff[d] + (int)std::min<size_t>(cdiv((size_t)v.dg.NX * v.dg.NY, 256), 256);
+    }
+    b->nitems = (int)items.size();
+    b->ioff[n] = b->nitems;
+    for (int d = 0; d < n; d++) { bd[d].item_lo = b->ioff[d]; bd[d].item_hi = b->ioff[d + 1]; }
+    b->eblocks = eoff[n]; b->hblocks = hoff[n]; b->sblocks = soff[n];
+    void *p;
+    int rc;
+    if ((rc = balloc(b, &p, 2 * sizeof(double) * (size_t)b->nitems))) return rc;
+    b->d_dtpart = (double *)p;
+    for (auto &a : descs) a.dtpart = b->d_dtpart;
+    if ((rc = balloc(b, &p, sizeof(FArgs<float>) * descs.size()))) return rc;
+    b->d_descs = (FArgs<float> *)p;
+    if ((rc = balloc(b, &p, sizeof(int4) * items.size()))) return rc;
+    b->d_items = (int4 *)p;
+    if ((rc = balloc(b, &p, sizeof(BDisc) * n))) return rc;
+    b->d_bd = (BDisc *)p;
+    if ((rc = balloc(b, &p, sizeof(int) * (n + 1)))) return rc;
+    b->d_eoff = (int *)p;
+    if ((rc = balloc(b, &p, sizeof(int) * (n + 1)))) return rc;
+    b->d_hoff = (int *)p;
+    if ((rc = balloc(b, &p, sizeof(int) * (n + 1)))) return rc;
+    b->d_soff = (int *)p;
+    if ((rc = balloc(b, &p, sizeof(int) * (n + 1)))) return rc;
+    b->d_ioff = (int *)p;
+    BCHK(hipMemcpy(b->d_descs, descs.data(), sizeof(FArgs<float>) * descs.size(), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(b->d_bd, bd.data(), sizeof(BDisc) * n, hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(b->d_eoff, eoff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(b->d_hoff, hoff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(b->d_soff, soff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(b->d_ioff, b->ioff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    b->built = true;
+    b->primed = false;
+    return CYLMHD_OK;
+}
+
+template <bool MC, bool HLL>
+void launch_stage(cylmhd_batch *b, int s)
+{
+    const FArgs<float> *descs = b->d_descs + (size_t)s * b->n;
+    hipLaunchKernelGGL((k_edge_batch<MC, HLL>), dim3(b->eblocks), dim3(256), 0, b->stream, descs, b->d_eoff, b->n, b->d_bd);
+    const int G = std::max(8, std::min((b->nitems + 7) / 8 * 8, b->grid));
+    if (b->shape)
+        hipLaunchKernelGGL((k_fused_batch<MC, HLL, 1>), dim3(G), dim3(tile_threads<float, 1>()), 0, b->stream, descs,
+                           b->d_items, b->nitems, b->d_bd);
+    else
+        hipLaunchKernelGGL((k_fused_batch<MC, HLL, 0>), dim3(G), dim3(tile_threads<float, 0>()), 0, b->stream, descs,
+                           b->d_items, b->nitems, b->d_bd);
+    hipLaunchKernelGGL(k_halo_batch, dim3(b->hblocks), dim3(256), 0, b->stream, descs, b->d_hoff, b->n, b->d_bd);
+}
+
+/* one main.c step for every disc: Setdt, SpiralGravity, TVDRK */
+void enqueue_step(cylmhd_batch *b)
+{
+    hipLaunchKernelGGL(k_dt_batch, dim3(b->n), dim3(256), 0, b->stream, b->d_descs, b->nst, b->n, b->d_bd, b->d_dtpart);
+    hipLaunchKernelGGL(k_sincos_batch, dim3(b->sblocks), dim3(256), 0, b->stream, b->d_descs, b->d_soff, b->n, b->d_bd);
+    for (int s = 0; s < b->nst; s++) {
+        if (b->mc) { if (b->hll) launch_stage<true, true>(b, s); else launch_stage<true, false>(b, s); }
+        else       { if (b->hll) launch_stage<false, true>(b, s); else launch_stage<false, false>(b, s); }
+    }
+    hipLaunchKernelGGL(k_time_batch, dim3(cdiv(b->n, 64)), dim3(64), 0, b->stream, b->d_bd, b->n);
+}
+
+/* Setdt partials of the stored states, floor counters cleared */
+int prime(cylmhd_batch *b)
+{
+    for (int d = 0; d < b->n; d++) {
+        BCHK(hipStreamSynchronize(b->view[d].stream));
+        BCHK(hipMemsetAsync(b->view[d].ds.counts, 0, 9 * sizeof(int), b->stream));
+    }
+    hipLaunchKernelGGL(k_setdt_batch, dim3(b->nitems), dim3(256), 0, b->stream, b->d_descs, b->d_bd, b->d_ioff, b->n,
+                       b->d_dtpart);
+    BCHK(hipGetLastError());
+    b->primed = true;
+    return CYLMHD_OK;
+}
+
+int capture(cylmhd_batch *b)
+{
+    free_graph(b);
+    BCHK(hipStreamSynchronize(b->stream));
+    BCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
+    enqueue_step(b);
+    hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+    if (e != hipSuccess) { set_errmsg("hipStreamEndCapture failed"); return CYLMHD_EHIP; }
+    BCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return CYLMHD_OK;
+}
+
+int backup(cylmhd_batch *b, bool save)
+{
+    size_t off = 0;
+    for (int d = 0; d < b->n; d++) {
+        const size_t cnt = 9 * b->view[d].fs->NS;
+        float *un = (float *)b->view[d].fs->Un;
+        if (save) BCHK(hipMemcpyAsync(b->d_backup + off, un, cnt * sizeof(float), hipMemcpyDeviceToDevice, b->stream));
+        else BCHK(hipMemcpyAsync(un, b->d_backup + off, cnt * sizeof(float), hipMemcpyDeviceToDevice, b->stream));
+        off += cnt;
+    }
+    return CYLMHD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out)
+{
+    if (!p || n < 1 || !out) return CYLMHD_EINVAL;
+    *out = nullptr;
+    for (int d = 0; d < n; d++) {
+        if (p[d].path != CYLMHD_PATH_FUSED32 || p[d].limiter != p[0].limiter || p[d].solver != p[0].solver ||
+            p[d].rk != p[0].rk || p[d].limiter > CYLMHD_LIMITER_MC) {
+            set_errmsg("batch: every disc must use the FP32 fused path and the same limiter (minmod/MC), solver and RK");
+            return CYLMHD_EINVAL;
+        }
+    }
+    cylmhd_batch *b = new cylmhd_batch();
+    b->n = n;
+    b->mc = p[0].limiter == CYLMHD_LIMITER_MC;
+    b->hll = p[0].solver == CYLMHD_SOLVER_HLL;
+    b->rk3 = p[0].rk != 2;
+    b->nst = b->rk3 ? 3 : 2;
+    int rc = CYLMHD_OK;
+    size_t tot = 0;
+    for (int d = 0; d < n && !rc; d++) {
+        cylmhd_ctx *c = nullptr;
+        rc = cylmhd_create(&p[d], &c);
+        if (rc) break;
+        b->disc.push_back(c);
+        DiscView v;
+        rc = disc_view(c, &v);
+        b->view.push_back(v);
+        tot += 9 * v.fs->NS;
+    }
+    if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) rc = CYLMHD_EHIP;
+    if (!rc && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)) rc = CYLMHD_EHIP;
+    if (!rc) {
+        void *q;
+        rc = balloc(b, &q, tot * sizeof(float));
+        if (!rc) {
+            b->d_backup = (float *)q;
+            b->backup_elems = tot;
+            b->allocs.clear();          /* the backup outlives work-list rebuilds */
+        }
+    }
+    if (!rc) rc = build(b, false);
+    if (rc) {
+        cylmhd_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return CYLMHD_OK;
+}
+
+int cylmhd_batch_destroy(cylmhd_batch *b)
+{
+    if (!b) return CYLMHD_OK;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    free_lists(b);
+    if (b->d_backup) hipFree(b->d_backup);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->stream) hipStreamDestroy(b->stream);
+    for (cylmhd_ctx *c : b->disc) cylmhd_destroy(c);
+    delete b;
+    return CYLMHD_OK;
+}
+
+int cylmhd_batch_disc(cylmhd_batch *b, int d, cylmhd_ctx **ctx)
+{
+    if (!b || !ctx || d < 0 || d >= b->n) return CYLMHD_EINVAL;
+    *ctx = b->disc[d];
+    b->primed = false;            /* the caller may change the state */
+    return CYLMHD_OK;
+}
+
+int cylmhd_batch_run(cylmhd_batch *b, int nsteps, double *ms)
+{
+    if (!b || nsteps < 0) return CYLMHD_EINVAL;
+    int rc;
+    if (!b->primed && (rc = prime(b))) return rc;
+    if (!b->exec && (rc = capture(b))) return rc;
+    BCHK(hipEventRecord(b->ev0, b->stream));
+    for (int k = 0; k < nsteps; k++) BCHK(hipGraphLaunch(b->exec, b->stream));
+    BCHK(hipEventRecord(b->ev1, b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
+    if (ms) {
+        float f = 0.0f;
+        BCHK(hipEventElapsedTime(&f, b->ev0, b->ev1));
+        *ms = f;
+    }
+    for (cylmhd_ctx *c : b->disc) disc_invalidate_dt(c);
+    return CYLMHD_OK;
+}
+
+int cylmhd_batch_status(cylmhd_batch *b, int d, double *t, double *dt, int *steps, int *alive)
+{
+    if (!b || d < 0 || d >= b->n) return CYLMHD_EINVAL;
+    BDisc D;
+    BCHK(hipMemcpy(&D, b->d_bd + d, sizeof D, hipMemcpyDeviceToHost));
+    if (t) *t = D.t;
+    if (dt) *dt = D.dt;
+    if (steps) *steps = D.steps;
+    if (alive) *alive = D.alive;
+    return CYLMHD_OK;
+}
+
+int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
+{
+    if (!b || steps < 1) return CYLMHD_EINVAL;
+    static const int shapes[2] = {0, 1};
+    static const int seglens[4] = {8, 16, 32, 0};
+    static const int grids[2] = {512, 1024};
+    int rc;
+    if ((rc = backup(b, true))) return rc;
+    BCHK(hipStreamSynchronize(b->stream));
+    double best = 1e300;
+    int bs = 0, bl = 0, bg = 1024;
+    for (int si = 0; si < 2; si++)
+        for (int li = 0; li < 4; li++)
+            for (int gi = 0; gi < 2; gi++) {
+                b->shape = shapes[si];
+                b->seglen = seglens[li];
+                b->grid = grids[gi];
+                if ((rc = build(b, false)) || (rc = prime(b))) return rc;
+                enqueue_step(b);                      /* warm-up */
+                BCHK(hipEventRecord(b->ev0, b->stream));
+                for (int k = 0; k < steps; k++) enqueue_step(b);
+                BCHK(hipEventRecord(b->ev1, b->stream));
+                BCHK(hipStreamSynchronize(b->stream));
+                float f = 0.0f;
+                BCHK(hipEventElapsedTime(&f, b->ev0, b->ev1));
+                if (f < best) { best = f; bs = shapes[si]; bl = seglens[li]; bg = grids[gi]; }
+                if ((rc = backup(b, false))) return rc;
+            }
+    BCHK(hipStreamSynchronize(b->stream));
+    b->shape = bs; b->seglen = bl; b->grid = bg;
+    if ((rc = build(b, false))) return rc;
+    if (choice) { choice[0] = bs; choice[1] = bl; choice[2] = bg; }
+    return CYLMHD_OK;
+}
+
+int cylmhd_batch_floor_counts(cylmhd_batch *b, int d, int counts[9])
+{
+    if (!b || d < 0 || d >= b->n) return CYLMHD_EINVAL;
+    BCHK(hipStreamSynchronize(b->stream));
+    return cylmhd_floor_counts(b->disc[d], counts);
+}
+
+}  // extern "C"

@@ -24,6 +24,10 @@ struct FusedState {
     int dt_blocks = 0;
     bool dt_valid = false;
     int nseg = 1, seglen = 1, tiles_j = 1, tiles_k = 1;
+    /* padded layout of the state arrays: component stride NS, k rows of
+     * NZP cells with k = 1 at offset 16 (64-byte aligned FP32 tile rows) */
+    size_t NS = 0;
+    int NZP = 0, NZ = 0;
     /* geometry packed for the stage kernel (few base pointers):
      *   itab [NX][IT] FP64  x, xm, oneox, oneodx, xabs, spiral r factors
      *   ikg  [NX*NZ][4] T   g1, g2, g3, g5

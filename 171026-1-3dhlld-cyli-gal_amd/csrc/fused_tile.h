@@ -1,0 +1,691 @@
+/*
+ * fused_tile.h -- device side of the fused stage (internal): the work-item
+ * function of one RK stage over a (phi, z) tile column, the r-boundary faces
+ * and the ghost-cell map.  Included by fused.hip (single disc; built once per
+ * state precision) and batch.hip (many discs in one launch).  The includer
+ * defines FUSED_NS and FT before including it.
+ *
+ * The stage on one tile marches through r.  At plane i a 3-slot LDS ring
+ * holds the primitives of planes i, i+1, i+2 (each plane is loaded and
+ * converted UtoV once) plus the phi/z halo of plane i, and
+ *   - x: computes the states of cell i+1 (one limiter call gives both of its
+ *        faces) and the r-face flux at i+1/2 (the one at i-1/2 is carried
+ *        in registers from the previous plane);
+ *   - phi, z: the limiter states of every cell of the tile and its one-cell
+ *        halo, once per direction, into LDS; then the faces (HLLD/HLL);
+ *   - accumulates the divergence in the reference's order
+ *        ((g2 F+ - g1 F-) + (g4 G+ - g3 G-)) + (g6 H+ - g5 H-),
+ *     adds SpiralGravityCooling, forms the RK stage combination, applies
+ *     the absorbing layers (last stage), Check, and stores U;
+ *   - writes the pre-Check values of the cells the boundary map reads into
+ *     a shadow array (for the ghost pass), and on the last stage the Setdt
+ *     partial.
+ * Reads per cell and stage: U_in once (plus the halo, mostly L2 hits) and
+ * once more at the update, U^n on stages 2-3; writes U_out once.  V, the
+ * face fluxes and S never touch HBM.
+ *
+ * Against the plain path the only change in the arithmetic is that V is
+ * recomputed from the stored U each stage instead of being carried from
+ * the previous Check (in floored cells the two differ in the last bits).
+ * FP32 state keeps the pressure recovery of UtoV, the rebuild of the energy
+ * in Check, the RK combination, the CFL reduction and the diagnostics in
+ * FP64.
+ *
+ * Boundary faces: phi is periodic in both the cell and the interface
+ * boundaries (YVlrPeriodic), so halo rows wrap (-1 -> NY-3, NY -> 2); the r
+ * faces 0 and NX-2 both take (upper(NX-2), lower(1)) (XVlrPeriodic) and are
+ * computed by edge_cell; the z faces 0 and NZ-2 follow
+ * ZLeftVlVzAntisymBzsym / ZRightVrFree (minmod) or ZVlrFixed (MC).
+ */
+#ifndef CYLMHD_FUSED_TILE_H
+#define CYLMHD_FUSED_TILE_H
+
+#include <climits>
+
+#include "fused.h"
+
+namespace cyl {
+namespace FUSED_NS {
+
+using FT = FUSED_FT;
+
+/* C3: cell index in the reference layout (Uinitial, the dt argmax);
+ * SI: cell index in the state arrays of this path, whose k rows are padded
+ * so that k = 1 (the first interior cell) starts a 64-byte line: a tile row
+ * of 16 FP32 cells is then one aligned 64-byte access. */
+#define C3(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * g.NZ + (size_t)(k))
+#define SI(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * (size_t)a.NZP + (size_t)(k) + KOFF)
+
+
+/* ---- precision-specific UtoV and Check ---------------------------------- */
+template <typename T> struct Prec;
+
+template <> struct Prec<double> {
+    static __device__ __forceinline__ void utov(const Consts &c, const double *U, double oneox, double *V)
+    {
+        cyl::utov<double>(c, U, oneox, V);
+    }
+    static __device__ __forceinline__ int check(const Consts &c, double *U, double *V, double x, double rf)
+    {
+        return cyl::check<double>(c, U, V, x, rf);
+    }
+};
+
+/* FP32 state: p = gm1 (E - (rho v)^2/2rho - B^2/2) and the energy rebuild
+ * of Check are formed in FP64 (the cancellation points of low-beta and
+ * high-Mach cells); the rest stays FP32. */
+template <> struct Prec<float> {
+    static __device__ __forceinline__ void utov(const Consts &c, const float *U, double oneox, float *V)
+    {
+        const float oneorho = 1.0f / U[0];
+        const double d1 = U[1], d2 = U[2], d3 = U[3], d4 = U[4], d5 = U[5], d6 = U[6];
+        const double b2 = d4 * d4 + d5 * d5 + d6 * d6;
+        const double r2v2 = d1 * d1 + d2 * d2 * oneox * oneox + d3 * d3;
+        V[0] = U[0];
+        V[1] = U[1] * oneorho;
+        V[2] = U[2] * oneorho * (float)oneox;
+        V[3] = U[3] * oneorho;
+        V[4] = U[4];
+        V[5] = U[5];
+        V[6] = U[6];
+        V[7] = (float)(c.gm1 * ((double)U[7] - 0.5 * (r2v2 / (double)U[0] + b2)));
+        V[8] = U[8];
+    }
+    static __device__ __forceinline__ int check(const Consts &c, float *U, float *V, double x, float rf)
+    {
+        int flags = 0;
+        const double v2 = (double)V[2] * V[2];
+        const double b2 = ((double)V[4] * V[4] + (double)V[5] * V[5] + (double)V[6] * V[6]);
+        if (V[0] < rf) {
+            flags |= 1;
+            V[0] = rf; V[1] = 0.0f; V[3] = 0.0f;
+            U[0] = V[0]; U[1] = 0.0f; U[2] = (float)(x * V[0] * V[2]); U[3] = 0.0f;
+            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7]);
+        }
+        const float prfloor = V[0] * (float)c.prcr;
+        if (V[7] < prfloor) {
+            flags |= 2;
+            V[1] = 0.0f; V[3] = 0.0f; V[7] = prfloor;
+            U[1] = 0.0f; U[3] = 0.0f;
+            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7]);
+        }
+        const double oneobeta = 0.5 * b2 / (double)V[7];
+        if (oneobeta > c.oneobetacr) {
+            flags |= 4;
+            V[0] = (float)(c.betacr * oneobeta * V[0]); V[1] = 0.0f; V[3] = 0.0f;
+            V[7] = (float)(c.betacr * oneobeta * V[7]);
+            U[0] = V[0]; U[1] = 0.0f; U[2] = (float)(x * V[0] * V[2]); U[3] = 0.0f;
+            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7]);
+        }
+        return flags;
+    }
+};
+
+/* packed geometry rows (FusedState::itab, ktab) */
+enum { IT_X, IT_XM, IT_OX, IT_ODX, IT_XABS, IT_X2S2, IT_O1X2S2, IT_XS2O15, IT_X2S2O15, IT = 12 };
+enum { KT_ODZ, KT_Z, KT_SZS, KT_ONEPA, KT_OZ2S2, KT_ZABS, KT = 8 };
+
+template <typename T> struct FArgs {
+    DevGeom g;
+    Consts c;
+    const double *itab, *ikd, *ktab;
+    const T *ikg;
+    const T *Uin, *Un;
+    T *Uout, *shadow;
+    const T *rhofloor;
+    const double *Uinit, *Vinit_z;
+    size_t NS;                 /* component stride of the state arrays */
+    int NZP;                   /* padded k-row length of the state arrays */
+    const T *edge;
+    int *counts;
+    double *dtpart;
+    size_t N;
+    int kind, seglen, tiles_j, tiles_k, ntiles, nblocks;
+    bool last, rk3, want_dt;
+};
+
+__device__ __forceinline__ int jmap(const DevGeom &g, int jg)
+{
+    if (jg == -1) return g.NY - 3;                  /* periodic wrap of the phi halo */
+    if (jg == g.NY) return 2;
+    return jg < 0 ? 0 : (jg > g.NY - 1 ? g.NY - 1 : jg);
+}
+__device__ __forceinline__ int kmap(const DevGeom &g, int kg) { return kg < 0 ? 0 : (kg > g.NZ - 1 ? g.NZ - 1 : kg); }
+
+/* stage input of one cell as primitives: psi damped (GLMSourceSolver) then UtoV */
+template <typename T>
+__device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, T *v)
+{
+    T u[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) u[m] = a.Uin[m * a.NS + cc];
+    u[8] = T(a.c.psicoef) * u[8];
+    Prec<T>::utov(a.c, u, a.itab[(size_t)i * IT + IT_OX], v);
+}
+
+/* XCD-aware block order: consecutive tiles (which share halo planes) go to
+ * the same XCD's L2 (blocks are dealt round-robin over the 8 XCDs).  The
+ * grid is a multiple of 8, so this is a bijection of [0, gridDim.x). */
+__device__ __forceinline__ int remap_block(int b, int nb)
+{
+    const int per = nb / 8;
+    return (b % 8) * per + b / 8;
+}
+
+/* ---- packed geometry ---------------------------------------------------- */
+template <typename T>
+__global__ void k_pack(DevGeom g, double *itab, T *ikg, double *ikd, double *ktab)
+{
+    const size_t nik = (size_t)g.NX * g.NZ;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < nik; q += (size_t)gridDim.x * blockDim.x) {
+        ikg[4 * q + 0] = T(g.g1[q]); ikg[4 * q + 1] = T(g.g2[q]); ikg[4 * q + 2] = T(g.g3[q]); ikg[4 * q + 3] = T(g.g5[q]);
+        ikd[4 * q + 0] = g.minlength[q]; ikd[4 * q + 1] = g.grxaxi[q]; ikd[4 * q + 2] = g.grzaxi[q]; ikd[4 * q + 3] = g.sp_q[q];
+        if (q < (size_t)g.NX) {
+            const int i = (int)q;
+            double *r = itab + (size_t)i * IT;
+            r[IT_X] = g.x[i]; r[IT_XM] = g.xm[i]; r[IT_OX] = g.oneox[i]; r[IT_ODX] = g.oneodx[i];
+            r[IT_XABS] = i >= g.NX - 11 ? g.xabs[i - (g.NX - 11)] : 0.0;
+            r[IT_X2S2] = g.sp_x2s2[i]; r[IT_O1X2S2] = g.sp_o1x2s2[i];
+            r[IT_XS2O15] = g.sp_xs2o15[i]; r[IT_X2S2O15] = g.sp_x2s2o15[i];
+        }
+        if (q < (size_t)g.NZ) {
+            const int k = (int)q;
+            double *r = ktab + (size_t)k * KT;
+            r[KT_ODZ] = g.oneodz[k]; r[KT_Z] = g.z[k]; r[KT_SZS] = g.sp_szs[k];
+            r[KT_ONEPA] = g.sp_onepa[k]; r[KT_OZ2S2] = g.sp_oz2s2[k];
+            r[KT_ZABS] = k >= g.NZ - 11 ? g.zabs[k - (g.NZ - 11)] : 0.0;
+        }
+    }
+}
+
+/* gravity_cell (ctx.h) from the packed rows, same expression order */
+template <typename T>
+__device__ __forceinline__ bool gravity_packed(const FArgs<T> &a, int i, int j, const double *it, const double *kt,
+                                               const double *ikd, double &gx, double &gy, double &gz)
+{
+    const DevGeom &g = a.g;
+    if (i < g.irin) {
+        gx = ikd[1];
+        gy = 0.0;
+        gz = ikd[2];
+        return false;
+    }
+    const double m1 = 10.0;
+    const double epssp = g.epssp;
+    const double q = ikd[3];
+    const double szs = kt[KT_SZS];
+    const double sinphase = g.sincos[(size_t)i * g.NY + j];
+    const double cosphase = g.sincos[(size_t)g.NX * g.NY + (size_t)i * g.NY + j];
+    const double x = it[IT_X], z = kt[KT_Z];
+    const double spiralfr = (-epssp * m1 * q * szs * it[IT_XS2O15]
+                             * ((x * q * q - ((2.0 - it[IT_X2S2]) * it[IT_O1X2S2])) * cosphase
+                                + g.armnum * g.cotpitch * sinphase));
+    const double spiralfphi = (epssp * m1 * q * szs * it[IT_XS2O15] * g.armnum * sinphase);
+    const double spiralfz = (-epssp * m1 * q * z * szs * it[IT_X2S2O15]
+                             * (kt[KT_ONEPA] * q * q + kt[KT_OZ2S2]) * cosphase);
+    gx = ikd[1] + spiralfr;
+    gy = 0.0 + spiralfphi;
+    gz = ikd[2] + spiralfz;
+    return true;
+}
+
+/* Tile shapes: a block owns TJ x TK (phi x z) cells and has (TJ+2) x TK
+ * threads, one per phi-state task (the TJ rows plus one halo row each
+ * side): FP32 14 x 16 / 256 threads, FP64 6 x 16 / 128 threads.  The LDS
+ * of one block (3-plane ring + phi and z state buffers) is 75 KB (FP32) or
+ * 77 KB (FP64), so two blocks share a CU. */
+constexpr int KOFF = 15;     /* k = 1 at offset 16 of a padded row */
+
+template <typename T, int SHAPE> struct Tile;
+template <> struct Tile<float, 0>  { static constexpr int TJ = 14, TK = 16, WAVES = 2; };
+template <> struct Tile<double, 0> { static constexpr int TJ = 6, TK = 16, WAVES = 1; };
+/* a wider-in-z shape for small discs (the batch autotuner picks one) */
+template <> struct Tile<float, 1>  { static constexpr int TJ = 6, TK = 32, WAVES = 2; };
+template <> struct Tile<double, 1> { static constexpr int TJ = 6, TK = 16, WAVES = 1; };
+template <typename T, int SHAPE> constexpr int tile_threads() { return (Tile<T, SHAPE>::TJ + 2) * Tile<T, SHAPE>::TK; }
+template <typename T, int SHAPE> constexpr int tile_waves() { return Tile<T, SHAPE>::WAVES; }
+
+/* One RK stage on the work item (tile, segment): the r-march over planes
+ * ibeg..iend of a TJ x TK (phi x z) column.  slot indexes the Setdt
+ * partial the item writes on the last stage. */
+template <typename T, bool MC, bool HLL, int SHAPE>
+__device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg, int slot)
+{
+    constexpr int TJ = Tile<T, SHAPE>::TJ, TK = Tile<T, SHAPE>::TK;
+    constexpr int NTH = (TJ + 2) * TK;           /* threads = phi-state tasks */
+    constexpr int NZT = TJ * (TK + 2);           /* z-state tasks */
+    constexpr int NCELL = TJ * TK, NHALO = 4 * TK + 4 * TJ;
+    static_assert(NZT <= NTH && TJ * (TK + 1) <= NTH && NHALO <= NTH, "tile does not fit the block");
+    constexpr int LIM = MC ? 1 : 0;
+    /* padded plane: (TJ+4) x (TK+4) cells (corners unused), 9 components,
+     * 3 ring slots; every stencil read is a per-thread base + an immediate */
+    constexpr int PK = TK + 4, PS = (TJ + 4) * PK, SLOT = 9 * PS;
+    const DevGeom &g = a.g;
+    const Consts &c = a.c;
+    __shared__ T ring[3 * SLOT];
+    /* limiter states of plane i: phi direction, task q = (row + 1) * TK + col
+     * for rows -1..TJ; z direction, task q = row * (TK + 2) + col + 1 for
+     * cols -1..TK.  [0] upper (+face) state, [1] lower (-face) state; the
+     * face pass overwrites a cell's lower state with the flux through its
+     * -face (only that face reads it). */
+    __shared__ __align__(16) T sy[2][9 * NTH];
+    __shared__ T sz[2][9 * NZT];
+
+    const int t = threadIdx.x;
+    const int tj = tile / a.tiles_k, tk = tile % a.tiles_k;
+    const int j0 = 1 + tj * TJ, k0 = 1 + tk * TK;
+    const int ibeg = 1 + seg * a.seglen;
+    const int iend = min(g.NX - 2, ibeg + a.seglen - 1);
+    if (ibeg > iend) {                            /* empty segment: neutral Setdt partial */
+        if (a.want_dt && t == 0) { a.dtpart[2 * slot] = 0.0; a.dtpart[2 * slot + 1] = 9.3e18; }
+        return;
+    }
+    __syncthreads();                              /* LDS of a previous work item is free */
+    const bool is_cell = t < NCELL;
+    const int cj = t / TK, ck = t % TK;                 /* cell coordinates in the tile */
+    const int gj = j0 + cj, gk = k0 + ck;
+    const bool valid = is_cell && gj <= g.NY - 2 && gk <= g.NZ - 2;
+    const size_t N = a.N, NS = a.NS;
+    const int cbase = (cj + 2) * PK + ck + 2;           /* own cell in a padded plane */
+    /* phi-state task: row yr (-1..TJ), col yc */
+    const int yr = t / TK - 1, yc = t % TK;
+    const int ybase = (yr + 1) * PK + yc + 2;           /* rows yr-1..yr+1 of col yc */
+    /* z-state task: row zr, col zc (-1..TK) */
+    const bool zstate = t < NZT;
+    const int zr = t / (TK + 2), zc = t % (TK + 2) - 1;
+    const int zbase = (zr + 2) * PK + zc + 1;           /* cols zc-1..zc+1 of row zr */
+    const int zcg = kmap(g, k0 + zc);
+    /* faces: phi face (fyj, fyk) between rows fyj-1 and fyj; z face
+     * (fzj, fzk) between cols fzk-1 and fzk, global face index kf */
+    const bool yface = t < (TJ + 1) * TK;
+    const int fyj = t / TK, fyk = t % TK;
+    const bool zface = t < TJ * (TK + 1);
+    const int fzj = t / (TK + 1), fzk = t % (TK + 1);
+    const int kf = k0 + fzk - 1;
+
+    auto ld9 = [&](int off, T *v) {
+#pragma unroll
+        for (int m = 0; m < 9; m++) v[m] = ring[off + m * PS];
+    };
+    auto load_center = [&](int p) {
+        if (is_cell && p >= 0 && p <= g.NX - 1) {
+            T v[9];
+            cell_V(a, SI(p, jmap(g, gj), kmap(g, gk)), p, v);
+            const int off = (p % 3) * SLOT + cbase;
+#pragma unroll
+            for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
+        }
+    };
+    int hj = 0, hk = 0;
+    if (t < 2 * TK) { hj = -2 + t / TK; hk = t % TK; }
+    else if (t < 4 * TK) { hj = TJ + (t - 2 * TK) / TK; hk = (t - 2 * TK) % TK; }
+    else if (t < 4 * TK + 2 * TJ) { const int u = t - 4 * TK; hj = u / 2; hk = -2 + u % 2; }
+    else { const int u = t - 4 * TK - 2 * TJ; hj = u / 2; hk = TK + u % 2; }
+    const size_t hcol = (size_t)jmap(g, j0 + hj) * a.NZP + kmap(g, k0 + hk) + KOFF;
+    const int hoff = (hj + 2) * PK + hk + 2;
+    auto load_halo = [&](int p) {
+        if (t < NHALO) {
+            T v[9];
+            cell_V(a, (size_t)p * g.NY * a.NZP + hcol, p, v);
+            const int off = (p % 3) * SLOT + hoff;
+#pragma unroll
+            for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
+        }
+    };
+
+    T up_prev[9], fx_prev[9], fxn[9];
+    int f0 = 0, f1 = 0, f2 = 0;
+    double best = 0.0;
+    long long bix = LLONG_MAX;
+    const size_t NE = (size_t)g.NY * g.NZ;
+
+    /* ---- prologue: upper state of cell ibeg and the flux at ibeg - 1/2 --- */
+    if (ibeg == 1) {
+        load_center(0); load_center(1); load_center(2);
+        __syncthreads();
+        if (is_cell) {
+            T vm[9], v0[9], vp[9];
+            ld9(0 * SLOT + cbase, vm); ld9(1 * SLOT + cbase, v0); ld9(2 * SLOT + cbase, vp);
+            cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[1 * IT + IT_ODX]), T(a.itab[0 * IT + IT_ODX]), T(1.0), up_prev, (T *)nullptr);
+            const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
+#pragma unroll
+            for (int m = 0; m < 9; m++) fx_prev[m] = a.edge[m * NE + e];   /* face 0 */
+        }
+    } else {
+        T upm[9];
+        load_center(ibeg - 2); load_center(ibeg - 1); load_center(ibeg);
+        __syncthreads();
+        if (is_cell) {
+            T vm[9], v0[9], vp[9];
+            ld9(((ibeg - 2) % 3) * SLOT + cbase, vm); ld9(((ibeg - 1) % 3) * SLOT + cbase, v0);
+            ld9((ibeg % 3) * SLOT + cbase, vp);
+            cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[(ibeg - 1) * IT + IT_ODX]), T(a.itab[(ibeg - 2) * IT + IT_ODX]), T(1.0), upm, (T *)nullptr);
+        }
+        __syncthreads();
+        load_center(ibeg + 1);
+        __syncthreads();
+        if (is_cell) {
+            T vm[9], v0[9], vp[9], lo[9];
+            ld9(((ibeg - 1) % 3) * SLOT + cbase, vm); ld9((ibeg % 3) * SLOT + cbase, v0);
+            ld9(((ibeg + 1) % 3) * SLOT + cbase, vp);
+            cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[ibeg * IT + IT_ODX]), T(a.itab[(ibeg - 1) * IT + IT_ODX]), T(1.0), up_prev, lo);
+            riemann<0, HLL>(c, upm, lo, T(a.itab[(ibeg - 1) * IT + IT_XM]), T(a.itab[(ibeg - 1) * IT + IT_OX]), fx_prev);
+        }
+    }
+
+    /* No barrier is needed before the loads of plane i: the center of
+     * plane i+2 goes into the slot of plane i-1, whose cells other threads
+     * last read in the state pass of iteration i-1 (before its face
+     * barrier), and the halo of plane i goes into halo positions that only
+     * the state pass of iteration i reads. */
+    for (int i = ibeg; i <= iend; i++) {
+        if (i + 2 <= g.NX - 1 && i < g.NX - 2) load_center(i + 2);
+        load_halo(i);
+        __syncthreads();
+        const int s0 = (i % 3) * SLOT, s1 = ((i + 1) % 3) * SLOT, s2 = ((i + 2) % 3) * SLOT;
+        const double *it = a.itab + (size_t)i * IT;
+        /* ---- x: flux at i+1/2, divergence term of the r faces ------------ */
+        if (is_cell) {
+            if (i == g.NX - 2) {
+                const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
+#pragma unroll
+                for (int m = 0; m < 9; m++) fxn[m] = a.edge[9 * NE + m * NE + e];   /* face NX-2 */
+            } else {
+                T vm[9], v0[9], vp[9], lo[9], up[9];
+                ld9(s0 + cbase, vm); ld9(s1 + cbase, v0); ld9(s2 + cbase, vp);
+                cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[(i + 1) * IT + IT_ODX]), T(it[IT_ODX]), T(1.0), up, lo);
+                riemann<0, HLL>(c, up_prev, lo, T(it[IT_XM]), T(it[IT_OX]), fxn);
+#pragma unroll
+                for (int m = 0; m < 9; m++) up_prev[m] = up[m];
+            }
+        }
+        /* ---- limiter states, once per cell and direction ---------------- */
+        {
+            T vm[9], v0[9], vp[9], up[9], lo[9];
+            const T f1 = T(it[IT_OX]), f2 = T(g.oneody);
+            ld9(s0 + ybase, vm); ld9(s0 + ybase + PK, v0); ld9(s0 + ybase + 2 * PK, vp);
+            cell_states<1, LIM>(c, vm, v0, vp, f1, f1, f2, up, lo);
+#pragma unroll
+            for (int m = 0; m < 9; m++) { sy[0][m * NTH + t] = up[m]; sy[1][m * NTH + t] = lo[m]; }
+        }
+        if (zstate) {
+            T vm[9], v0[9], vp[9], up[9], lo[9];
+            ld9(s0 + zbase, vm); ld9(s0 + zbase + 1, v0); ld9(s0 + zbase + 2, vp);
+            cell_states<2, LIM>(c, vm, v0, vp, T(a.ktab[zcg * KT + KT_ODZ]), T(a.ktab[(zcg > 0 ? zcg - 1 : 0) * KT + KT_ODZ]), T(1.0), up, lo);
+#pragma unroll
+            for (int m = 0; m < 9; m++) { sz[0][m * NZT + t] = up[m]; sz[1][m * NZT + t] = lo[m]; }
+        }
+        __syncthreads();
+        /* ---- phi faces: (TJ+1) x TK ------------------------------------- */
+        if (yface) {
+            T vl[9], vr[9], fy[9];
+            const int qu = fyj * TK + fyk, ql = (fyj + 1) * TK + fyk;
+#pragma unroll
+            for (int m = 0; m < 9; m++) { vl[m] = sy[0][m * NTH + qu]; vr[m] = sy[1][m * NTH + ql]; }
+            riemann<1, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fy);
+#pragma unroll
+            for (int m = 0; m < 9; m++) sy[1][m * NTH + ql] = fy[m];
+        }
+        /* ---- z faces: TJ x (TK+1) --------------------------------------- */
+        if (zface && kf <= g.NZ - 2) {
+            T vl[9], vr[9], fz[9];
+            const int qu = fzj * (TK + 2) + fzk, ql = qu + 1;
+            const int jgl = jmap(g, j0 + fzj);
+            if (kf == 0) {
+                if (MC) {
+                    /* ZVlrFixedBoundary: Vinitial at k = 0 */
+                    const size_t NP = (size_t)g.NX * g.NY;
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vl[m] = T(a.Vinit_z[(size_t)m * 2 * NP + (size_t)i * g.NY + jgl]);
+                } else {
+                    /* ZLeftVlVzAntisymBzsym: lower state of cell 2, vz negated */
+                    T vm[9], v0[9], vp[9];
+                    const int zb = s0 + (fzj + 2) * PK + 2;       /* cols 0..2 of the tile = k 1..3 */
+                    ld9(zb, vm); ld9(zb + 1, v0); ld9(zb + 2, vp);
+                    cell_states<2, LIM>(c, vm, v0, vp, T(a.ktab[2 * KT + KT_ODZ]), T(a.ktab[1 * KT + KT_ODZ]), T(1.0), (T *)nullptr, vl);
+                    vl[3] = -vl[3];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 9; m++) vl[m] = sz[0][m * NZT + qu];
+            }
+            if (kf == g.NZ - 2) {
+                if (MC) {
+                    /* ZVlrFixedBoundary: Vinitial at k = NZ-1 */
+                    const size_t NP = (size_t)g.NX * g.NY;
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vr[m] = T(a.Vinit_z[(size_t)m * 2 * NP + NP + (size_t)i * g.NY + jgl]);
+                } else {
+                    /* ZRightVrFreeBoundary: Vr = Vl */
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vr[m] = vl[m];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 9; m++) vr[m] = sz[1][m * NZT + ql];
+            }
+            riemann<2, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fz);
+#pragma unroll
+            for (int m = 0; m < 9; m++) sz[1][m * NZT + ql] = fz[m];
+        }
+        /* inputs of the update, fetched before the barrier so that their
+         * latency overlaps it (set on every lane: dead again after use) */
+        const size_t cc = SI(i, gj, gk), ccr = C3(i, gj, gk);
+        T uin[9], unn[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) { uin[m] = T(0); unn[m] = T(0); }
+        if (valid) {
+#pragma unroll
+            for (int m = 0; m < 9; m++) uin[m] = a.Uin[m * NS + cc];
+            if (a.kind == 2 || a.kind == 3 || a.kind == 5) {
+#pragma unroll
+                for (int m = 0; m < 9; m++) unn[m] = a.Un[m * NS + cc];
+            }
+        }
+        __syncthreads();
+        /* ---- divergence, sources, stage update, absorb, Check, store ----- */
+        if (valid) {
+            const size_t ik = (size_t)i * g.NZ + gk;
+            T acc[9];
+            {
+                const T g1 = a.ikg[4 * ik + 0], g2 = a.ikg[4 * ik + 1];
+#pragma unroll
+                for (int m = 0; m < 9; m++) { acc[m] = g2 * fxn[m] - g1 * fx_prev[m]; fx_prev[m] = fxn[m]; }
+            }
+            {
+                const T g3 = a.ikg[4 * ik + 2], g4 = g3;
+                const int qm = (cj + 1) * TK + ck, qp = qm + TK;
+#pragma unroll
+                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g4 * sy[1][m * NTH + qp] - g3 * sy[1][m * NTH + qm]);
+            }
+            {
+                const T g5 = a.ikg[4 * ik + 3], g6 = g5;
+                const int qm = cj * (TK + 2) + ck + 1, qp = qm + 1;
+#pragma unroll
+                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz[1][m * NZT + qp] - g5 * sz[1][m * NZT + qm]);
+            }
+            T v[9], S[9], u[9], un[9];
+            ld9(s0 + cbase, v);
+            const double *kt = a.ktab + (size_t)gk * KT;
+            double gx, gy, gz;
+            const bool spiral = gravity_packed(a, i, gj, it, kt, a.ikd + 4 * ik, gx, gy, gz);
+            sources(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
+#pragma unroll
+            for (int m = 0; m < 9; m++) u[m] = uin[m];
+            u[8] = T(c.psicoef) * u[8];
+            const int kind = a.kind;
+            if (kind == 2 || kind == 3 || kind == 5) {
+#pragma unroll
+                for (int m = 0; m < 9; m++) un[m] = unn[m];
+                un[8] = T(c.psicoef) * un[8];
+            }
+            /* The RK combination is formed in FP64 and rounded once: with FP32
+             * state the stage-2/3 forms would otherwise round four or five
+             * times per update (for FP64 state this is the same arithmetic). */
+            const double dt = c.dt, oneo3 = 1.0 / 3.0;
+#pragma unroll
+            for (int m = 0; m < 9; m++) {
+                const double div = acc[m], um = u[m], unm = un[m], sm = S[m];
+                double r;
+                if (kind == 1 || kind == 4) r = um - dt * div + dt * sm;
+                else if (kind == 2) r = 0.25 * (3.0 * unm + um - dt * div + dt * sm);
+                else if (kind == 3) r = oneo3 * (unm + 2.0 * (um - dt * div + dt * sm));
+                else r = 0.5 * (unm + um - dt * div + dt * sm);
+                u[m] = T(r);
+            }
+            if (!a.rk3 && gk == 1) u[3] = T(0.0);       /* ZLeftVzAntisymBSym, U3[k=1] */
+            if (i == 1 || i == g.NX - 2 || gj == 1 || gj == g.NY - 2 || gk <= 2 || gk == g.NZ - 2) {
+#pragma unroll
+                for (int m = 0; m < 9; m++) a.shadow[m * NS + cc] = u[m];
+            }
+            if (a.last) {
+                const bool ax = i >= g.NX - 11, ac = i <= g.irin, az = !a.rk3 && gk >= g.NZ - 11;
+                if (ax || ac || az) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) {
+                        const T ui = T(a.Uinit[m * N + ccr]);
+                        if (ax) u[m] -= (T(it[IT_XABS]) * (u[m] - ui));
+                        if (ac) u[m] -= (T(g.cabs[(size_t)i * g.NZ + gk]) * (u[m] - ui));
+                        if (az) u[m] -= (T(kt[KT_ZABS]) * (u[m] - ui));
+                    }
+                }
+            }
+            Prec<T>::utov(c, u, it[IT_OX], v);
+            const int fl = Prec<T>::check(c, u, v, it[IT_X], a.rhofloor[cc]);
+            f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
+#pragma unroll
+            for (int m = 0; m < 9; m++) a.Uout[m * NS + cc] = u[m];
+            if (a.want_dt && i < g.NX - 11) {
+                /* Setdt (setdt.h:33-69) on the checked primitives, FP64 */
+                const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
+                const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
+                const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
+                const double t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / a.ikd[4 * ik + 0];
+                const double tc = (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7)));
+                const double tt = rmax(t1, tc);
+                const long long ix = (long long)ccr;
+                if ((tt > best || (tt == best && ix < bix)) && tt > 0.0) { best = tt; bix = ix; }
+            }
+        } else if (is_cell) {
+#pragma unroll
+            for (int m = 0; m < 9; m++) fx_prev[m] = fxn[m];
+        }
+    }
+    /* ---- floor counters: one atomic per wave ---------------------------- */
+    for (int off = 32; off > 0; off >>= 1) {
+        f0 += __shfl_xor(f0, off); f1 += __shfl_xor(f1, off); f2 += __shfl_xor(f2, off);
+    }
+    if ((t & 63) == 0) {
+        if (f0) atomicAdd(&a.counts[0], f0);
+        if (f1) atomicAdd(&a.counts[1], f1);
+        if (f2) atomicAdd(&a.counts[2], f2);
+    }
+    if (a.want_dt) {
+        __syncthreads();
+        double *sv = (double *)&sy[0][0];
+        long long *sx = (long long *)(sv + NTH);
+        sv[t] = best; sx[t] = bix;
+        __syncthreads();
+        for (int off = NTH / 2; off > 0; off >>= 1) {
+            if (t < off) {
+                const double v = sv[t + off];
+                const long long ix = sx[t + off];
+                if (v > sv[t] || (v == sv[t] && ix < sx[t])) { sv[t] = v; sx[t] = ix; }
+            }
+            __syncthreads();
+        }
+        if (t == 0) { a.dtpart[2 * slot] = sv[0]; a.dtpart[2 * slot + 1] = (double)sx[0]; }
+    }
+}
+
+/* ---- r-boundary faces 0 and NX-2: (upper(NX-2), lower(1)) ---------------- */
+template <typename T, bool MC, bool HLL>
+__device__ __forceinline__ void edge_cell(const FArgs<T> &a, T *edge, int q)
+{
+    const DevGeom &g = a.g;
+    const int nj = g.NY - 2, nk = g.NZ - 2;
+    if (q >= nj * nk) return;
+    const int j = 1 + q / nk, k = 1 + q % nk;
+    T vm[9], v0[9], vp[9], up[9], lo[9], f[9];
+    const int L = g.NX - 2;
+    cell_V(a, SI(L - 1, j, k), L - 1, vm); cell_V(a, SI(L, j, k), L, v0); cell_V(a, SI(L + 1, j, k), L + 1, vp);
+    cell_states<0, MC>(a.c, vm, v0, vp, T(g.oneodx[L]), T(g.oneodx[L - 1]), T(1.0), up, (T *)nullptr);
+    cell_V(a, SI(0, j, k), 0, vm); cell_V(a, SI(1, j, k), 1, v0); cell_V(a, SI(2, j, k), 2, vp);
+    cell_states<0, MC>(a.c, vm, v0, vp, T(g.oneodx[1]), T(g.oneodx[0]), T(1.0), (T *)nullptr, lo);
+    const size_t NE = (size_t)g.NY * g.NZ, e = (size_t)j * g.NZ + k;
+    riemann<0, HLL>(a.c, up, lo, T(g.xm[0]), T(g.oneox[0]), f);
+#pragma unroll
+    for (int m = 0; m < 9; m++) edge[m * NE + e] = f[m];
+    riemann<0, HLL>(a.c, up, lo, T(g.xm[L]), T(g.oneox[L]), f);
+#pragma unroll
+    for (int m = 0; m < 9; m++) edge[9 * NE + m * NE + e] = f[m];
+}
+
+/* ---- ghost cells: boundary map of the shadow, then absorb / UtoV / Check */
+template <typename T>
+__device__ __forceinline__ size_t halo_cells(const DevGeom &g)
+{
+    const size_t NX = g.NX, NY = g.NY, NZ = g.NZ;
+    return 2 * NY * NZ + (NX - 2) * 2 * NZ + (NX - 2) * (NY - 2) * 2;
+}
+
+/* ghost cells first, first + stride, ... of the composed boundary map;
+ * every lane of the wave must call it (the counters are wave-reduced) */
+template <typename T>
+__device__ __forceinline__ void halo_range(const FArgs<T> &a, size_t first, size_t stride)
+{
+    const DevGeom &g = a.g;
+    const size_t NX = g.NX, NY = g.NY, NZ = g.NZ, N = a.N, NS = a.NS;
+    const size_t nA = 2 * NY * NZ, nB = (NX - 2) * 2 * NZ, nC = (NX - 2) * (NY - 2) * 2;
+    const size_t tot = nA + nB + nC;
+    int f0 = 0, f1 = 0, f2 = 0;
+    for (size_t t = first; t < tot; t += stride) {
+        int i, j, k;
+        size_t r = t;
+        if (r < nA) { i = (r / (NY * NZ)) ? (int)NX - 1 : 0; r %= NY * NZ; j = (int)(r / NZ); k = (int)(r % NZ); }
+        else if ((r -= nA) < nB) { i = 1 + (int)(r / (2 * NZ)); r %= 2 * NZ; j = (r / NZ) ? (int)NY - 1 : 0; k = (int)(r % NZ); }
+        else { r -= nB; i = 1 + (int)(r / ((NY - 2) * 2)); r %= (NY - 2) * 2; j = 1 + (int)(r / 2); k = (r % 2) ? (int)NZ - 1 : 0; }
+        int si, sj, sk;
+        unsigned sign;
+        bool zero3;
+        ghost_source(g, a.rk3, i, j, k, si, sj, sk, sign, zero3);
+        const size_t src = SI(si, sj, sk), dst = SI(i, j, k), dstr = C3(i, j, k);
+        T u[9], v[9];
+#pragma unroll
+        for (int m = 0; m < 9; m++) {
+            const T val = a.shadow[m * NS + src];
+            u[m] = (sign >> m) & 1u ? -val : val;
+        }
+        if (zero3) u[3] = T(0.0);
+        if (a.last) {
+            const bool ax = i >= g.NX - 11, ac = i <= g.irin, az = !a.rk3 && k >= g.NZ - 11;
+            if (ax || ac || az) {
+#pragma unroll
+                for (int m = 0; m < 9; m++) {
+                    const T ui = T(a.Uinit[m * N + dstr]);
+                    if (ax) u[m] -= (T(g.xabs[i - (g.NX - 11)]) * (u[m] - ui));
+                    if (ac) u[m] -= (T(g.cabs[(size_t)i * g.NZ + k]) * (u[m] - ui));
+                    if (az) u[m] -= (T(g.zabs[k - (g.NZ - 11)]) * (u[m] - ui));
+                }
+            }
+        }
+        Prec<T>::utov(a.c, u, g.oneox[i], v);
+        const int fl = Prec<T>::check(a.c, u, v, g.x[i], a.rhofloor[dst]);
+        f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
+#pragma unroll
+        for (int m = 0; m < 9; m++) a.Uout[m * NS + dst] = u[m];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        f0 += __shfl_xor(f0, off); f1 += __shfl_xor(f1, off); f2 += __shfl_xor(f2, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (f0) atomicAdd(&a.counts[0], f0);
+        if (f1) atomicAdd(&a.counts[1], f1);
+        if (f2) atomicAdd(&a.counts[2], f2);
+    }
+}
+
+
+}  // namespace FUSED_NS
+}  // namespace cyl
+#endif

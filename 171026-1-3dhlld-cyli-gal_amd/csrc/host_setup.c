@@ -121,6 +121,26 @@ int cylmhd_params_config(cylmhd_params *p, int config, int nr, int nphi, int nz)
     return CYLMHD_OK;
 }
 
+static double u01(uint64_t seed, uint64_t a, uint64_t b, uint64_t c);
+
+int cylmhd_params_batch(cylmhd_params *p, int n, uint64_t seed)
+{
+    static const int dims[5] = {32, 48, 64, 96, 128};
+    if (!p || n < 1) return CYLMHD_EINVAL;
+    for (int d = 0; d < n; d++) {
+        /* draw 1: nr, 2: nphi, 3: nz among the dims not above nr, 4: spiral */
+        const int ir = (int)(5.0 * u01(seed, (uint64_t)d, 1, 0));
+        const int nr = dims[ir], nphi = dims[(int)(5.0 * u01(seed, (uint64_t)d, 2, 0))];
+        const int nz = dims[(int)((double)(ir + 1) * u01(seed, (uint64_t)d, 3, 0))];
+        int rc = cylmhd_params_config(&p[d], 4, nr, nphi, nz);
+        if (rc) return rc;
+        p[d].seed = seed + 7919ULL * (uint64_t)(d + 1);
+        p[d].epssp0 = 0.1 * u01(seed, (uint64_t)d, 4, 0);
+        p[d].path = CYLMHD_PATH_FUSED32;
+    }
+    return CYLMHD_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 void cyl_grid_free(cyl_grid *g)
 {

@@ -116,6 +116,15 @@ def lib():
         "cylmhd_binary_output": ([C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double,
                                   _dp, _dp, _dp, _dp], C.c_int),
         "cylmhd_restart_read": ([C.c_char_p, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
+        "cylmhd_params_batch": ([P, C.c_int, C.c_uint64], C.c_int),
+        "cylmhd_batch_create": ([P, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+        "cylmhd_batch_destroy": ([C.c_void_p], C.c_int),
+        "cylmhd_batch_disc": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+        "cylmhd_batch_tune": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
+        "cylmhd_batch_run": ([C.c_void_p, C.c_int, C.POINTER(C.c_double)], C.c_int),
+        "cylmhd_batch_status": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+        "cylmhd_batch_floor_counts": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
     }
     for name, (args, res) in sig.items():
         try:
@@ -297,3 +306,81 @@ def exported_symbols():
     with open(hdr) as f:
         txt = f.read()
     return sorted(set(re.findall(r"\b(cylmhd_[a-z0-9_]+)\s*\(", txt)))
+
+
+def params_batch(n, seed=20171026):
+    """Parameters of the benchmark batch (cylmhd_params_batch)."""
+    arr = (Params * n)()
+    check(lib().cylmhd_params_batch(arr, n, seed), "params_batch")
+    return [arr[d].copy() for d in range(n)]
+
+
+class _DiscRef(Context):
+    """A disc context owned by a Batch (not destroyed from Python)."""
+
+    def __init__(self, p, h):
+        self.p = p.copy()
+        self.dims = p.dims
+        self.h = h
+
+    def close(self):
+        self.h = None
+
+
+class Batch:
+    """Many discs advanced together (cylmhd_batch_*): one work list, one
+    launch per RK stage, per-disc dt on the device, the step replayed from a
+    captured hipGraph."""
+
+    def __init__(self, params):
+        self.params = [p.copy() for p in params]
+        n = len(params)
+        arr = (Params * n)(*self.params)
+        h = C.c_void_p()
+        check(lib().cylmhd_batch_create(arr, n, C.byref(h)), "batch_create")
+        self.h = h
+        self.n = n
+
+    def disc(self, d):
+        h = C.c_void_p()
+        check(lib().cylmhd_batch_disc(self.h, d, C.byref(h)), "batch_disc")
+        return _DiscRef(self.params[d], h)
+
+    def tune(self, steps=2):
+        ch = (C.c_int * 3)()
+        check(lib().cylmhd_batch_tune(self.h, steps, ch), "batch_tune")
+        return tuple(ch)
+
+    def run(self, nsteps):
+        ms = C.c_double()
+        check(lib().cylmhd_batch_run(self.h, nsteps, C.byref(ms)), "batch_run")
+        return ms.value
+
+    def status(self, d):
+        t, dt = C.c_double(), C.c_double()
+        st, al = C.c_int(), C.c_int()
+        check(lib().cylmhd_batch_status(self.h, d, C.byref(t), C.byref(dt), C.byref(st), C.byref(al)),
+              "batch_status")
+        return dict(t=t.value, dt=dt.value, steps=st.value, alive=bool(al.value))
+
+    def floor_counts(self, d):
+        c = (C.c_int * 9)()
+        check(lib().cylmhd_batch_floor_counts(self.h, d, c), "batch_floor_counts")
+        return np.array(list(c)).reshape(3, 3)
+
+    def close(self):
+        if self.h:
+            lib().cylmhd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

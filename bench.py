@@ -17,6 +17,9 @@ roofline  = the fused stage kernel: algorithmic HBM bytes per launch
 precision = relative L1 / Linf error per conserved variable of the FP32
             path against the plain FP64 path after 100 steps on the same
             dt sequence, with div B and floor hits (precision.py).
+batch     = the batching form: 48 uneven discs advanced together, one launch
+            per RK stage over a tile work list, per-disc dt on the device,
+            the step replayed from a hipGraph, launch shape autotuned.
 cpu_baseline = the CPU restatement of the reference (oracle, OpenMP on all
             host cores) or the reference itself (oracle/_ref bench build),
             on a bounded sample of the same disc.
@@ -83,6 +86,33 @@ def run_path(cm, np, path, grid, steps, warmup):
     return res
 
 
+def run_batch(cm, steps, n=48, seed=20171026):
+    """The batching form: n discs (dims from {32..128}, nz <= nr, beta
+    [1, 100], spiral [0, 0.1]) advanced together; autotuned, then `steps`
+    graph-replayed steps timed with HIP events."""
+    ps = cm.params_batch(n, seed)
+    cells = sum(p.ix * p.jy * p.kz for p in ps)
+    t0 = time.perf_counter()
+    b = cm.Batch(ps)
+    for d, p in enumerate(ps):
+        U, V = cm.disc_state(p)
+        b.disc(d).set_initial(U, V)
+    log(f"[bench] batch: {n} discs, {cells / 1e6:.2f} M cells, set-up {time.perf_counter() - t0:.1f}s")
+    choice = b.tune(2)
+    b.run(2)                                   # warm-up (graph capture)
+    ms = b.run(steps)
+    st = [b.status(d) for d in range(n)]
+    alive = sum(1 for x in st if x["alive"])
+    done = sum(x["steps"] for x in st)
+    b.close()
+    updates = sum(p.ix * p.jy * p.kz * (x["steps"] - 2) for p, x in zip(ps, st))
+    res = dict(discs=n, interior_cells=cells, steps=steps, ms=ms, value=updates / (ms * 1e-3),
+               unit="cell-updates/s", alive=alive, steps_done=done,
+               tuned={"shape": ["14x16", "6x32"][choice[0]], "segment": choice[1] or "whole r", "grid": choice[2]})
+    log(f"[bench] batch: {res['value']:.4g} cell-updates/s over {steps} steps ({ms:.1f} ms), tuned {res['tuned']}")
+    return res
+
+
 def cpu_baseline(cm, np, budget_s=15.0):
     """CPU restatement of the reference on a bounded sample: the same disc at
     128x128x64, as many whole steps as fit the time budget (at least 1)."""
@@ -113,6 +143,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--precision-steps", type=int, default=100,
                     help="steps of the FP32-vs-FP64 error report (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=200,
+                    help="steps of the 48-disc batching form (0: skip)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); the path would shard as "
@@ -168,6 +200,11 @@ def main():
                                                  cm.PATH_PLAIN64, cm.PATH_FUSED32, log=log)
         except Exception as e:                     # reported, never fatal to the GPU number
             out["precision"] = {"error": str(e)}
+    if a.batch_steps > 0:
+        try:
+            out["batch"] = run_batch(cm, a.batch_steps)
+        except Exception as e:                     # reported, never fatal to the GPU number
+            out["batch"] = {"error": str(e)}
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cm, np)

@@ -166,6 +166,38 @@ int cylmhd_sync(cylmhd_ctx *c);
 int cylmhd_kernel_timing(cylmhd_ctx *c, int which, int reset, double *avg_ms, int *launches);
 const char *cylmhd_error_string(int code);
 
+/* ---- batching form: many independent discs in one launch per stage ----
+ * Not in the reference (it runs one disc per process).  Every disc keeps a
+ * full context (create with CYLMHD_PATH_FUSED32), reachable through
+ * cylmhd_batch_disc for set-up, dumps and diagnostics; the batch advances
+ * them together: one work list of (disc, tile, r-segment) items, one
+ * persistent launch per RK stage over it, per-disc Setdt on the device (each
+ * disc its own dt; a disc whose dt falls below dtmin stops, the others go
+ * on), and the whole step captured once into a hipGraph and replayed. */
+typedef struct cylmhd_batch cylmhd_batch;
+
+/* Parameters of the benchmark batch: n discs, each dimension drawn from
+ * {32, 48, 64, 96, 128} with nz <= nr, beta log-uniform in [1, 100] per
+ * annulus, spiral amplitude uniform in [0, 0.1], its own seed; minmod +
+ * HLLD, RK3, FP32 path.  p must hold n entries. */
+int cylmhd_params_batch(cylmhd_params *p, int n, uint64_t seed);
+/* All discs must share limiter (minmod or MC), solver and RK order. */
+int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out);
+int cylmhd_batch_destroy(cylmhd_batch *b);
+/* The context of disc d (owned by the batch; do not destroy it). */
+int cylmhd_batch_disc(cylmhd_batch *b, int d, cylmhd_ctx **ctx);
+/* Autotuner: times `steps` steps for each tile shape x r-segment length x
+ * launch grid on the current states, restores them, keeps the fastest.
+ * choice (optional, 3 ints) = shape, segment length (0 = whole r), grid. */
+int cylmhd_batch_tune(cylmhd_batch *b, int steps, int choice[3]);
+/* Advance every live disc nsteps steps (graph replays); *ms (optional) =
+ * device time of the replays from HIP events on the batch stream. */
+int cylmhd_batch_run(cylmhd_batch *b, int nsteps, double *ms);
+/* Time, last dt, steps taken and whether disc d is still running. */
+int cylmhd_batch_status(cylmhd_batch *b, int d, double *t, double *dt, int *steps, int *alive);
+/* Check() floor hits of disc d summed over the steps of the last runs. */
+int cylmhd_batch_floor_counts(cylmhd_batch *b, int d, int counts[9]);
+
 /* ---- dump / restart byte format (host only) ---------------------------- */
 /* BinaryOutput (output.h:1202-1414): ./<dir>/GridTime.dat + r, phi, z,
  * rho, vr, vphi, vz, br, bphi, bz, pr, psi as raw native-endian FP64. */
