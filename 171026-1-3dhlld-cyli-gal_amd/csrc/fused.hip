@@ -7,6 +7,7 @@
 #include "fused.h"
 
 #include <algorithm>
+#include <cmath>
 
 /* This file is compiled twice: FUSED_PREC=64 (FP64 state, -ffp-contract=off
  * like the plain path) and FUSED_PREC=32 (FP32 state; contraction into FMAs
@@ -237,11 +238,25 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     fs.tiles_j = (g.NY - 2 + TJ - 1) / TJ;
     fs.tiles_k = (g.NZ - 2 + TK - 1) / TK;
     const int cols = fs.tiles_j * fs.tiles_k;
-    /* enough blocks to fill 256 CUs several times over, segments >= 8 planes */
-    int nseg = (2048 + cols - 1) / cols;
-    nseg = std::max(1, std::min(nseg, (g.NX - 2) / 8));
-    fs.seglen = (g.NX - 2 + nseg - 1) / nseg;
-    fs.nseg = (g.NX - 2 + fs.seglen - 1) / fs.seglen;
+    /* Segments of the r-march: the grid is dealt to the CUs in rounds of
+     * (CUs x 2 resident blocks); pick the count that wastes least in the
+     * last round, charging each segment its prologue (~2.5 planes), with
+     * segments of at least 8 planes. */
+    int ncu = 256;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || ncu < 1) ncu = 256;
+    const double slots = 2.0 * ncu;
+    const int planes = g.NX - 2;
+    int nseg = 1;
+    double bestc = 1e300;
+    for (int ns = 1; ns <= std::max(1, std::min(32, planes / 8)); ns++) {
+        const int sl = (planes + ns - 1) / ns;
+        const int nsr = (planes + sl - 1) / sl;
+        const double rounds = (double)cols * nsr / slots;
+        const double cost = std::ceil(rounds) / rounds * (sl + 2.5) / sl * (rounds < 1.0 ? 1.0 / rounds : 1.0);
+        if (cost < bestc - 1e-9) { bestc = cost; nseg = ns; }
+    }
+    fs.seglen = (planes + nseg - 1) / nseg;
+    fs.nseg = (planes + fs.seglen - 1) / fs.seglen;
     fs.dt_blocks = std::max((cols * fs.nseg + 7) / 8 * 8, 1024);
     if ((rc = alloc(2 * (size_t)fs.dt_blocks * sizeof(double), &p))) return rc;
     fs.dtpart = (double *)p;

@@ -145,15 +145,28 @@ def main():
                     help="steps of the FP32-vs-FP64 error report (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=200,
                     help="steps of the 48-disc batching form (0: skip)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run a plain device-to-device copy of this many MiB (the calibration "
+                         "kernel of the FETCH_SIZE / WRITE_SIZE counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {a.gpus}); the path would shard as "
-              f"phi-slabs with one-cell halo exchanges (see DESIGN.md)", file=sys.stderr)
+              f"phi-slabs with one-cell halo exchanges per stage", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import cylmhd as cm
 
     grid = tuple(a.grid)
+    if a.copy_mib > 0:
+        import torch
+        n = a.copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: 3 x {a.copy_mib} MiB device-to-device")
+        del src, dst
     main_res = run_path(cm, np, cm.PATH_FUSED32, grid, a.steps, a.warmup)
     fp64 = {}
     if not a.no_fp64:
