@@ -57,6 +57,18 @@ using FT = FUSED_FT;
 #define SI(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * (size_t)a.NZP + (size_t)(k) + KOFF)
 
 
+/* Loads from tables that no launch writes (geometry rows, packed source
+ * terms, boundary data) go through the constant address space: a
+ * wave-uniform index then becomes a scalar load instead of a vector load
+ * the wave has to wait for. */
+using cdp = const __attribute__((address_space(4))) double *;
+template <typename X>
+__device__ __forceinline__ X ro(const X *p, size_t idx)
+{
+    return ((const __attribute__((address_space(4))) X *)p)[idx];
+}
+
+
 /* ---- precision-specific UtoV and Check ---------------------------------- */
 template <typename T> struct Prec;
 
@@ -160,7 +172,22 @@ __device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, T *v
 #pragma unroll
     for (int m = 0; m < 9; m++) u[m] = a.Uin[m * a.NS + cc];
     u[8] = T(a.c.psicoef) * u[8];
-    Prec<T>::utov(a.c, u, a.itab[(size_t)i * IT + IT_OX], v);
+    Prec<T>::utov(a.c, u, ro(a.itab, (size_t)i * IT + IT_OX), v);
+}
+
+/* Work-group barrier for LDS traffic only.  __syncthreads() fences global
+ * memory as well, so every wave would wait at each barrier for all of its
+ * outstanding global loads, stores and direct-to-LDS loads (vmcnt 0).  The
+ * stage kernel's threads share data only through LDS, so here a wave waits
+ * only for its own LDS accesses (lgkmcnt 0: LDS is coherent within the
+ * work-group once an access has completed) and the compiler is kept from
+ * moving memory accesses across the barrier. */
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xc07f);          /* lgkmcnt(0); vmcnt, expcnt unconstrained */
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
 }
 
 /* XCD-aware block order: consecutive tiles (which share halo planes) go to
@@ -200,8 +227,8 @@ __global__ void k_pack(DevGeom g, double *itab, T *ikg, double *ikd, double *kta
 
 /* gravity_cell (ctx.h) from the packed rows, same expression order */
 template <typename T>
-__device__ __forceinline__ bool gravity_packed(const FArgs<T> &a, int i, int j, const double *it, const double *kt,
-                                               const double *ikd, double &gx, double &gy, double &gz)
+__device__ __forceinline__ bool gravity_packed(const FArgs<T> &a, int i, int j, cdp it, cdp kt,
+                                               cdp ikd, double &gx, double &gy, double &gz)
 {
     const DevGeom &g = a.g;
     if (i < g.irin) {
@@ -280,7 +307,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         if (a.want_dt && t == 0) { a.dtpart[2 * slot] = 0.0; a.dtpart[2 * slot + 1] = 9.3e18; }
         return;
     }
-    __syncthreads();                              /* LDS of a previous work item is free */
+    lds_barrier();                                /* LDS of a previous work item is free */
     const bool is_cell = t < NCELL;
     const int cj = t / TK, ck = t % TK;                 /* cell coordinates in the tile */
     const int gj = j0 + cj, gk = k0 + ck;
@@ -346,7 +373,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         if (is_cell) {
             T vm[9], v0[9], vp[9];
             ld9(0 * SLOT + cbase, vm); ld9(1 * SLOT + cbase, v0); ld9(2 * SLOT + cbase, vp);
-            cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[1 * IT + IT_ODX]), T(a.itab[0 * IT + IT_ODX]), T(1.0), up_prev, (T *)nullptr);
+            cell_states<0, LIM>(c, vm, v0, vp, T(ro(a.itab, 1 * IT + IT_ODX)), T(ro(a.itab, 0 * IT + IT_ODX)), T(1.0), up_prev, (T *)nullptr);
             const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
 #pragma unroll
             for (int m = 0; m < 9; m++) fx_prev[m] = a.edge[m * NE + e];   /* face 0 */
@@ -359,7 +386,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             T vm[9], v0[9], vp[9];
             ld9(((ibeg - 2) % 3) * SLOT + cbase, vm); ld9(((ibeg - 1) % 3) * SLOT + cbase, v0);
             ld9((ibeg % 3) * SLOT + cbase, vp);
-            cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[(ibeg - 1) * IT + IT_ODX]), T(a.itab[(ibeg - 2) * IT + IT_ODX]), T(1.0), upm, (T *)nullptr);
+            cell_states<0, LIM>(c, vm, v0, vp, T(ro(a.itab, (ibeg - 1) * IT + IT_ODX)), T(ro(a.itab, (ibeg - 2) * IT + IT_ODX)), T(1.0), upm, (T *)nullptr);
         }
         __syncthreads();
         load_center(ibeg + 1);
@@ -368,8 +395,8 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             T vm[9], v0[9], vp[9], lo[9];
             ld9(((ibeg - 1) % 3) * SLOT + cbase, vm); ld9((ibeg % 3) * SLOT + cbase, v0);
             ld9(((ibeg + 1) % 3) * SLOT + cbase, vp);
-            cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[ibeg * IT + IT_ODX]), T(a.itab[(ibeg - 1) * IT + IT_ODX]), T(1.0), up_prev, lo);
-            riemann<0, HLL>(c, upm, lo, T(a.itab[(ibeg - 1) * IT + IT_XM]), T(a.itab[(ibeg - 1) * IT + IT_OX]), fx_prev);
+            cell_states<0, LIM>(c, vm, v0, vp, T(ro(a.itab, ibeg * IT + IT_ODX)), T(ro(a.itab, (ibeg - 1) * IT + IT_ODX)), T(1.0), up_prev, lo);
+            riemann<0, HLL>(c, upm, lo, T(ro(a.itab, (ibeg - 1) * IT + IT_XM)), T(ro(a.itab, (ibeg - 1) * IT + IT_OX)), fx_prev);
         }
     }
 
@@ -381,9 +408,9 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
     for (int i = ibeg; i <= iend; i++) {
         if (i + 2 <= g.NX - 1 && i < g.NX - 2) load_center(i + 2);
         load_halo(i);
-        __syncthreads();
+        lds_barrier();
         const int s0 = (i % 3) * SLOT, s1 = ((i + 1) % 3) * SLOT, s2 = ((i + 2) % 3) * SLOT;
-        const double *it = a.itab + (size_t)i * IT;
+        const cdp it = (cdp)a.itab + (size_t)i * IT;
         /* ---- x: flux at i+1/2, divergence term of the r faces ------------ */
         if (is_cell) {
             if (i == g.NX - 2) {
@@ -393,7 +420,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             } else {
                 T vm[9], v0[9], vp[9], lo[9], up[9];
                 ld9(s0 + cbase, vm); ld9(s1 + cbase, v0); ld9(s2 + cbase, vp);
-                cell_states<0, LIM>(c, vm, v0, vp, T(a.itab[(i + 1) * IT + IT_ODX]), T(it[IT_ODX]), T(1.0), up, lo);
+                cell_states<0, LIM>(c, vm, v0, vp, T(ro(a.itab, (i + 1) * IT + IT_ODX)), T(it[IT_ODX]), T(1.0), up, lo);
                 riemann<0, HLL>(c, up_prev, lo, T(it[IT_XM]), T(it[IT_OX]), fxn);
 #pragma unroll
                 for (int m = 0; m < 9; m++) up_prev[m] = up[m];
@@ -411,11 +438,11 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         if (zstate) {
             T vm[9], v0[9], vp[9], up[9], lo[9];
             ld9(s0 + zbase, vm); ld9(s0 + zbase + 1, v0); ld9(s0 + zbase + 2, vp);
-            cell_states<2, LIM>(c, vm, v0, vp, T(a.ktab[zcg * KT + KT_ODZ]), T(a.ktab[(zcg > 0 ? zcg - 1 : 0) * KT + KT_ODZ]), T(1.0), up, lo);
+            cell_states<2, LIM>(c, vm, v0, vp, T(ro(a.ktab, zcg * KT + KT_ODZ)), T(ro(a.ktab, (zcg > 0 ? zcg - 1 : 0) * KT + KT_ODZ)), T(1.0), up, lo);
 #pragma unroll
             for (int m = 0; m < 9; m++) { sz[0][m * NZT + t] = up[m]; sz[1][m * NZT + t] = lo[m]; }
         }
-        __syncthreads();
+        lds_barrier();
         /* ---- phi faces: (TJ+1) x TK ------------------------------------- */
         if (yface) {
             T vl[9], vr[9], fy[9];
@@ -442,7 +469,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     T vm[9], v0[9], vp[9];
                     const int zb = s0 + (fzj + 2) * PK + 2;       /* cols 0..2 of the tile = k 1..3 */
                     ld9(zb, vm); ld9(zb + 1, v0); ld9(zb + 2, vp);
-                    cell_states<2, LIM>(c, vm, v0, vp, T(a.ktab[2 * KT + KT_ODZ]), T(a.ktab[1 * KT + KT_ODZ]), T(1.0), (T *)nullptr, vl);
+                    cell_states<2, LIM>(c, vm, v0, vp, T(ro(a.ktab, 2 * KT + KT_ODZ)), T(ro(a.ktab, 1 * KT + KT_ODZ)), T(1.0), (T *)nullptr, vl);
                     vl[3] = -vl[3];
                 }
             } else {
@@ -482,33 +509,33 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                 for (int m = 0; m < 9; m++) unn[m] = a.Un[m * NS + cc];
             }
         }
-        __syncthreads();
+        lds_barrier();
         /* ---- divergence, sources, stage update, absorb, Check, store ----- */
         if (valid) {
             const size_t ik = (size_t)i * g.NZ + gk;
             T acc[9];
             {
-                const T g1 = a.ikg[4 * ik + 0], g2 = a.ikg[4 * ik + 1];
+                const T g1 = ro(a.ikg, 4 * ik + 0), g2 = ro(a.ikg, 4 * ik + 1);
 #pragma unroll
                 for (int m = 0; m < 9; m++) { acc[m] = g2 * fxn[m] - g1 * fx_prev[m]; fx_prev[m] = fxn[m]; }
             }
             {
-                const T g3 = a.ikg[4 * ik + 2], g4 = g3;
+                const T g3 = ro(a.ikg, 4 * ik + 2), g4 = g3;
                 const int qm = (cj + 1) * TK + ck, qp = qm + TK;
 #pragma unroll
                 for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g4 * sy[1][m * NTH + qp] - g3 * sy[1][m * NTH + qm]);
             }
             {
-                const T g5 = a.ikg[4 * ik + 3], g6 = g5;
+                const T g5 = ro(a.ikg, 4 * ik + 3), g6 = g5;
                 const int qm = cj * (TK + 2) + ck + 1, qp = qm + 1;
 #pragma unroll
                 for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz[1][m * NZT + qp] - g5 * sz[1][m * NZT + qm]);
             }
             T v[9], S[9], u[9], un[9];
             ld9(s0 + cbase, v);
-            const double *kt = a.ktab + (size_t)gk * KT;
+            const cdp kt = (cdp)a.ktab + (size_t)gk * KT;
             double gx, gy, gz;
-            const bool spiral = gravity_packed(a, i, gj, it, kt, a.ikd + 4 * ik, gx, gy, gz);
+            const bool spiral = gravity_packed(a, i, gj, it, kt, (cdp)a.ikd + 4 * ik, gx, gy, gz);
             sources(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
 #pragma unroll
             for (int m = 0; m < 9; m++) u[m] = uin[m];
