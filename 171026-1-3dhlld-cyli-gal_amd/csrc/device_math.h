@@ -43,6 +43,14 @@ __device__ __noinline__ double raymond_A(double te0, double temperature)
     return 0.108e-37 * pow(te0 * temperature, 10.26);
 }
 
+/* Power-law cooling factor T^index (index != 1/2), out of line for the same
+ * reason: an inlined FP64 pow keeps its polynomial constants in registers
+ * across the whole r-march of the stage kernel. */
+__device__ __noinline__ double powerlaw_pow(double temperature, double index)
+{
+    return pow(temperature, index);
+}
+
 /* ---- Cooling (cooling.h:22-60) --------------------------------------- */
 template <typename T>
 __device__ __forceinline__ T cooling(const Consts &c, T rho, T pr)
@@ -70,7 +78,7 @@ __device__ __forceinline__ T cooling(const Consts &c, T rho, T pr)
         const T temperature = pr / rho;
         if (temperature > T(c.cool_tmin))
             return c.cool_index == 0.5 ? T(c.cool_lambda0) * sqrt(temperature)
-                                       : T(c.cool_lambda0) * T(pow(double(temperature), c.cool_index));
+                                       : T(c.cool_lambda0) * T(powerlaw_pow(double(temperature), c.cool_index));
         return T(0.0);
     }
     return T(0.0);

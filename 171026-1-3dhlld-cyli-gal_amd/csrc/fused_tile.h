@@ -225,12 +225,13 @@ __global__ void k_pack(DevGeom g, double *itab, T *ikg, double *ikd, double *kta
     }
 }
 
-/* gravity_cell (ctx.h) from the packed rows, same expression order */
-template <typename T>
-__device__ __forceinline__ bool gravity_packed(const FArgs<T> &a, int i, int j, cdp it, cdp kt,
-                                               cdp ikd, double &gx, double &gy, double &gz)
+/* gravity_cell (ctx.h) from the packed rows, same expression order:
+ * ikd = (Grx axisymmetric, Grz axisymmetric, q) of cell (i, k), kt = the
+ * k row (szs, z, 1 + a, z^2 s^2 term), sc = (sin, cos) of the arm phase
+ * at (i, j) */
+__device__ __forceinline__ bool gravity_staged(const DevGeom &g, int i, cdp it, const double *ikd, const double *kt,
+                                               const double *sc, double &gx, double &gy, double &gz)
 {
-    const DevGeom &g = a.g;
     if (i < g.irin) {
         gx = ikd[1];
         gy = 0.0;
@@ -240,16 +241,16 @@ __device__ __forceinline__ bool gravity_packed(const FArgs<T> &a, int i, int j, 
     const double m1 = 10.0;
     const double epssp = g.epssp;
     const double q = ikd[3];
-    const double szs = kt[KT_SZS];
-    const double sinphase = g.sincos[(size_t)i * g.NY + j];
-    const double cosphase = g.sincos[(size_t)g.NX * g.NY + (size_t)i * g.NY + j];
-    const double x = it[IT_X], z = kt[KT_Z];
+    const double szs = kt[0];
+    const double sinphase = sc[0];
+    const double cosphase = sc[1];
+    const double x = it[IT_X], z = kt[1];
     const double spiralfr = (-epssp * m1 * q * szs * it[IT_XS2O15]
                              * ((x * q * q - ((2.0 - it[IT_X2S2]) * it[IT_O1X2S2])) * cosphase
                                 + g.armnum * g.cotpitch * sinphase));
     const double spiralfphi = (epssp * m1 * q * szs * it[IT_XS2O15] * g.armnum * sinphase);
     const double spiralfz = (-epssp * m1 * q * z * szs * it[IT_X2S2O15]
-                             * (kt[KT_ONEPA] * q * q + kt[KT_OZ2S2]) * cosphase);
+                             * (kt[2] * q * q + kt[3]) * cosphase);
     gx = ikd[1] + spiralfr;
     gy = 0.0 + spiralfphi;
     gz = ikd[2] + spiralfz;
@@ -297,10 +298,30 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
      * -face (only that face reads it). */
     __shared__ __align__(16) T sy[2][9 * NTH];
     __shared__ T sz[2][9 * NZT];
+    /* table rows of the update, staged into LDS with the plane loads so
+     * that the update phase waits on no global load: per plane (slot i % 2)
+     * the (i, k) rows ikg, ikd, cabs and the (i, j) arm phase sin/cos; per
+     * work item the k rows (szs, z, 1 + a, z^2 s^2 term, z absorption) and
+     * the z-state 1/dz pair */
+    __shared__ T s_ikg[2][TK][4];
+    __shared__ double s_ikd[2][TK][4];
+    __shared__ double s_cabs[2][TK];
+    __shared__ double s_sc[2][TJ][2];
+    __shared__ double s_kt[TK][5];
+    __shared__ T s_odz[TK + 2][2];
+    /* MC z boundary (ZVlrFixedBoundary): Vinitial at k = 0 (side 0) and
+     * k = NZ-1 (side 1) for the tile's phi rows of plane i; staged at the
+     * top of the plane in tiles that touch a z boundary, read by the z-face
+     * pass before the update barrier, so one buffer serves */
+    __shared__ T s_vz[2][TJ][9];
+    static_assert(TK + TJ <= NTH - NCELL, "no spare threads for the table rows");
+    static_assert(2 * TJ * 9 <= NTH, "no threads for the z-boundary rows");
 
     const int t = threadIdx.x;
     const int tj = tile / a.tiles_k, tk = tile % a.tiles_k;
     const int j0 = 1 + tj * TJ, k0 = 1 + tk * TK;
+    const int kend = min(k0 + TK - 1, g.NZ - 2);      /* last interior k of the tile */
+    const bool zedge = k0 == 1 || kend == g.NZ - 2;   /* tile has a z-boundary face */
     const int ibeg = 1 + seg * a.seglen;
     const int iend = min(g.NX - 2, ibeg + a.seglen - 1);
     if (ibeg > iend) {                            /* empty segment: neutral Setdt partial */
@@ -308,6 +329,15 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         return;
     }
     lds_barrier();                                /* LDS of a previous work item is free */
+    if (t < TK) {
+        const cdp kr = (cdp)a.ktab + (size_t)min(k0 + t, g.NZ - 1) * KT;
+        s_kt[t][0] = kr[KT_SZS]; s_kt[t][1] = kr[KT_Z]; s_kt[t][2] = kr[KT_ONEPA];
+        s_kt[t][3] = kr[KT_OZ2S2]; s_kt[t][4] = kr[KT_ZABS];
+    } else if (t < 2 * TK + 2) {
+        const int zg = kmap(g, k0 + (t - TK) - 1);
+        s_odz[t - TK][0] = T(ro(a.ktab, (size_t)zg * KT + KT_ODZ));
+        s_odz[t - TK][1] = T(ro(a.ktab, (size_t)(zg > 0 ? zg - 1 : 0) * KT + KT_ODZ));
+    }
     const bool is_cell = t < NCELL;
     const int cj = t / TK, ck = t % TK;                 /* cell coordinates in the tile */
     const int gj = j0 + cj, gk = k0 + ck;
@@ -360,6 +390,21 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         }
     };
 
+    /* table rows of plane p, by the threads without a cell */
+    auto stage_rows = [&](int p) {
+        const int sl = p & 1, u = t - NCELL;
+        if (u >= 0 && u < TK) {
+            const size_t ik = (size_t)p * g.NZ + min(k0 + u, g.NZ - 1);
+#pragma unroll
+            for (int q = 0; q < 4; q++) { s_ikg[sl][u][q] = ro(a.ikg, 4 * ik + q); s_ikd[sl][u][q] = ro(a.ikd, 4 * ik + q); }
+            if (p <= g.irin) s_cabs[sl][u] = ro(g.cabs, ik);
+        } else if (u >= TK && u < TK + TJ) {
+            const int j = min(j0 + u - TK, g.NY - 1);
+            s_sc[sl][u - TK][0] = ro(g.sincos, (size_t)p * g.NY + j);
+            s_sc[sl][u - TK][1] = ro(g.sincos, (size_t)g.NX * g.NY + (size_t)p * g.NY + j);
+        }
+    };
+
     T up_prev[9], fx_prev[9], fxn[9];
     int f0 = 0, f1 = 0, f2 = 0;
     double best = 0.0;
@@ -408,24 +453,17 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
     for (int i = ibeg; i <= iend; i++) {
         if (i + 2 <= g.NX - 1 && i < g.NX - 2) load_center(i + 2);
         load_halo(i);
+        stage_rows(i);
+        if (MC && zedge) {
+            const int side = t / (TJ * 9), q = t % (TJ * 9), rj = q / 9, m = q % 9;
+            if (side < 2 && ((side == 0 && k0 == 1) || (side == 1 && kend == g.NZ - 2))) {
+                const size_t NP = (size_t)g.NX * g.NY;
+                s_vz[side][rj][m] = T(ro(a.Vinit_z, (size_t)m * 2 * NP + side * NP + (size_t)i * g.NY + jmap(g, j0 + rj)));
+            }
+        }
         lds_barrier();
         const int s0 = (i % 3) * SLOT, s1 = ((i + 1) % 3) * SLOT, s2 = ((i + 2) % 3) * SLOT;
         const cdp it = (cdp)a.itab + (size_t)i * IT;
-        /* ---- x: flux at i+1/2, divergence term of the r faces ------------ */
-        if (is_cell) {
-            if (i == g.NX - 2) {
-                const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
-#pragma unroll
-                for (int m = 0; m < 9; m++) fxn[m] = a.edge[9 * NE + m * NE + e];   /* face NX-2 */
-            } else {
-                T vm[9], v0[9], vp[9], lo[9], up[9];
-                ld9(s0 + cbase, vm); ld9(s1 + cbase, v0); ld9(s2 + cbase, vp);
-                cell_states<0, LIM>(c, vm, v0, vp, T(ro(a.itab, (i + 1) * IT + IT_ODX)), T(it[IT_ODX]), T(1.0), up, lo);
-                riemann<0, HLL>(c, up_prev, lo, T(it[IT_XM]), T(it[IT_OX]), fxn);
-#pragma unroll
-                for (int m = 0; m < 9; m++) up_prev[m] = up[m];
-            }
-        }
         /* ---- limiter states, once per cell and direction ---------------- */
         {
             T vm[9], v0[9], vp[9], up[9], lo[9];
@@ -438,7 +476,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         if (zstate) {
             T vm[9], v0[9], vp[9], up[9], lo[9];
             ld9(s0 + zbase, vm); ld9(s0 + zbase + 1, v0); ld9(s0 + zbase + 2, vp);
-            cell_states<2, LIM>(c, vm, v0, vp, T(ro(a.ktab, zcg * KT + KT_ODZ)), T(ro(a.ktab, (zcg > 0 ? zcg - 1 : 0) * KT + KT_ODZ)), T(1.0), up, lo);
+            cell_states<2, LIM>(c, vm, v0, vp, s_odz[zc + 1][0], s_odz[zc + 1][1], T(1.0), up, lo);
 #pragma unroll
             for (int m = 0; m < 9; m++) { sz[0][m * NZT + t] = up[m]; sz[1][m * NZT + t] = lo[m]; }
         }
@@ -457,13 +495,11 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         if (zface && kf <= g.NZ - 2) {
             T vl[9], vr[9], fz[9];
             const int qu = fzj * (TK + 2) + fzk, ql = qu + 1;
-            const int jgl = jmap(g, j0 + fzj);
             if (kf == 0) {
                 if (MC) {
                     /* ZVlrFixedBoundary: Vinitial at k = 0 */
-                    const size_t NP = (size_t)g.NX * g.NY;
 #pragma unroll
-                    for (int m = 0; m < 9; m++) vl[m] = T(a.Vinit_z[(size_t)m * 2 * NP + (size_t)i * g.NY + jgl]);
+                    for (int m = 0; m < 9; m++) vl[m] = s_vz[0][fzj][m];
                 } else {
                     /* ZLeftVlVzAntisymBzsym: lower state of cell 2, vz negated */
                     T vm[9], v0[9], vp[9];
@@ -479,9 +515,8 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             if (kf == g.NZ - 2) {
                 if (MC) {
                     /* ZVlrFixedBoundary: Vinitial at k = NZ-1 */
-                    const size_t NP = (size_t)g.NX * g.NY;
 #pragma unroll
-                    for (int m = 0; m < 9; m++) vr[m] = T(a.Vinit_z[(size_t)m * 2 * NP + NP + (size_t)i * g.NY + jgl]);
+                    for (int m = 0; m < 9; m++) vr[m] = s_vz[1][fzj][m];
                 } else {
                     /* ZRightVrFreeBoundary: Vr = Vl */
 #pragma unroll
@@ -495,15 +530,32 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
 #pragma unroll
             for (int m = 0; m < 9; m++) sz[1][m * NZT + ql] = fz[m];
         }
+        /* ---- x: flux at i+1/2 (computed after the phi and z faces, so that
+         * it is not held in registers through those passes) */
+        if (is_cell) {
+            if (i == g.NX - 2) {
+                const size_t e = (size_t)jmap(g, gj) * g.NZ + kmap(g, gk);
+#pragma unroll
+                for (int m = 0; m < 9; m++) fxn[m] = a.edge[9 * NE + m * NE + e];   /* face NX-2 */
+            } else {
+                T vm[9], v0[9], vp[9], lo[9], up[9];
+                ld9(s0 + cbase, vm); ld9(s1 + cbase, v0); ld9(s2 + cbase, vp);
+                cell_states<0, LIM>(c, vm, v0, vp, T(ro(a.itab, (i + 1) * IT + IT_ODX)), T(it[IT_ODX]), T(1.0), up, lo);
+                riemann<0, HLL>(c, up_prev, lo, T(it[IT_XM]), T(it[IT_OX]), fxn);
+#pragma unroll
+                for (int m = 0; m < 9; m++) up_prev[m] = up[m];
+            }
+        }
         /* inputs of the update, fetched before the barrier so that their
          * latency overlaps it (set on every lane: dead again after use) */
         const size_t cc = SI(i, gj, gk), ccr = C3(i, gj, gk);
-        T uin[9], unn[9];
+        T uin[9], unn[9], rfl = T(0);
 #pragma unroll
         for (int m = 0; m < 9; m++) { uin[m] = T(0); unn[m] = T(0); }
         if (valid) {
 #pragma unroll
             for (int m = 0; m < 9; m++) uin[m] = a.Uin[m * NS + cc];
+            rfl = a.rhofloor[cc];
             if (a.kind == 2 || a.kind == 3 || a.kind == 5) {
 #pragma unroll
                 for (int m = 0; m < 9; m++) unn[m] = a.Un[m * NS + cc];
@@ -512,53 +564,51 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         lds_barrier();
         /* ---- divergence, sources, stage update, absorb, Check, store ----- */
         if (valid) {
-            const size_t ik = (size_t)i * g.NZ + gk;
+            const int sl = i & 1;
             T acc[9];
             {
-                const T g1 = ro(a.ikg, 4 * ik + 0), g2 = ro(a.ikg, 4 * ik + 1);
+                const T g1 = s_ikg[sl][ck][0], g2 = s_ikg[sl][ck][1];
 #pragma unroll
                 for (int m = 0; m < 9; m++) { acc[m] = g2 * fxn[m] - g1 * fx_prev[m]; fx_prev[m] = fxn[m]; }
             }
             {
-                const T g3 = ro(a.ikg, 4 * ik + 2), g4 = g3;
+                const T g3 = s_ikg[sl][ck][2], g4 = g3;
                 const int qm = (cj + 1) * TK + ck, qp = qm + TK;
 #pragma unroll
                 for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g4 * sy[1][m * NTH + qp] - g3 * sy[1][m * NTH + qm]);
             }
             {
-                const T g5 = ro(a.ikg, 4 * ik + 3), g6 = g5;
+                const T g5 = s_ikg[sl][ck][3], g6 = g5;
                 const int qm = cj * (TK + 2) + ck + 1, qp = qm + 1;
 #pragma unroll
                 for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz[1][m * NZT + qp] - g5 * sz[1][m * NZT + qm]);
             }
             T v[9], S[9], u[9], un[9];
             ld9(s0 + cbase, v);
-            const cdp kt = (cdp)a.ktab + (size_t)gk * KT;
             double gx, gy, gz;
-            const bool spiral = gravity_packed(a, i, gj, it, kt, (cdp)a.ikd + 4 * ik, gx, gy, gz);
+            const bool spiral = gravity_staged(g, i, it, s_ikd[sl][ck], s_kt[ck], s_sc[sl][cj], gx, gy, gz);
             sources(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
 #pragma unroll
-            for (int m = 0; m < 9; m++) u[m] = uin[m];
+            for (int m = 0; m < 9; m++) { u[m] = uin[m]; un[m] = unn[m]; }
             u[8] = T(c.psicoef) * u[8];
-            const int kind = a.kind;
-            if (kind == 2 || kind == 3 || kind == 5) {
-#pragma unroll
-                for (int m = 0; m < 9; m++) un[m] = unn[m];
-                un[8] = T(c.psicoef) * un[8];
-            }
+            un[8] = T(c.psicoef) * un[8];             /* Un is zero in stages that do not read it */
             /* The RK combination is formed in FP64 and rounded once: with FP32
              * state the stage-2/3 forms would otherwise round four or five
-             * times per update (for FP64 state this is the same arithmetic). */
-            const double dt = c.dt, oneo3 = 1.0 / 3.0;
+             * times per update (for FP64 state this is the same arithmetic).
+             * One branch-free form covers the stage kinds (the coefficients
+             * are uniform), s = ((u + ka Un) - dt div) + dt S, r = kf (kb Un +
+             * ke s):  kinds 1, 4: s;  2: 0.25 ((3 Un + U) - dt div + dt S);
+             * 3: (1/3) (Un + 2 s);  5: 0.5 ((Un + U) - dt div + dt S). */
+            const int kind = a.kind;
+            const double dt = c.dt;
+            const double ka = kind == 2 ? 3.0 : (kind == 5 ? 1.0 : 0.0);
+            const double kb = kind == 3 ? 1.0 : 0.0, ke = kind == 3 ? 2.0 : 1.0;
+            const double kf = kind == 2 ? 0.25 : (kind == 3 ? 1.0 / 3.0 : (kind == 5 ? 0.5 : 1.0));
 #pragma unroll
             for (int m = 0; m < 9; m++) {
                 const double div = acc[m], um = u[m], unm = un[m], sm = S[m];
-                double r;
-                if (kind == 1 || kind == 4) r = um - dt * div + dt * sm;
-                else if (kind == 2) r = 0.25 * (3.0 * unm + um - dt * div + dt * sm);
-                else if (kind == 3) r = oneo3 * (unm + 2.0 * (um - dt * div + dt * sm));
-                else r = 0.5 * (unm + um - dt * div + dt * sm);
-                u[m] = T(r);
+                const double sv = ((um + ka * unm) - dt * div) + dt * sm;
+                u[m] = T(kf * (kb * unm + ke * sv));
             }
             if (!a.rk3 && gk == 1) u[3] = T(0.0);       /* ZLeftVzAntisymBSym, U3[k=1] */
             if (i == 1 || i == g.NX - 2 || gj == 1 || gj == g.NY - 2 || gk <= 2 || gk == g.NZ - 2) {
@@ -572,13 +622,13 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     for (int m = 0; m < 9; m++) {
                         const T ui = T(a.Uinit[m * N + ccr]);
                         if (ax) u[m] -= (T(it[IT_XABS]) * (u[m] - ui));
-                        if (ac) u[m] -= (T(g.cabs[(size_t)i * g.NZ + gk]) * (u[m] - ui));
-                        if (az) u[m] -= (T(kt[KT_ZABS]) * (u[m] - ui));
+                        if (ac) u[m] -= (T(s_cabs[sl][ck]) * (u[m] - ui));
+                        if (az) u[m] -= (T(s_kt[ck][4]) * (u[m] - ui));
                     }
                 }
             }
             Prec<T>::utov(c, u, it[IT_OX], v);
-            const int fl = Prec<T>::check(c, u, v, it[IT_X], a.rhofloor[cc]);
+            const int fl = Prec<T>::check(c, u, v, it[IT_X], rfl);
             f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
 #pragma unroll
             for (int m = 0; m < 9; m++) a.Uout[m * NS + cc] = u[m];
@@ -587,7 +637,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                 const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
                 const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
                 const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
-                const double t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / a.ikd[4 * ik + 0];
+                const double t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / s_ikd[sl][ck][0];
                 const double tc = (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7)));
                 const double tt = rmax(t1, tc);
                 const long long ix = (long long)ccr;
