@@ -45,8 +45,8 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(cm, np, path, grid, steps, warmup):
-    p = cm.params_config(3, *grid)
+def run_path(cm, np, path, grid, steps, warmup, config=3):
+    p = cm.params_config(config, *grid)
     p.path = path
     t0 = time.perf_counter()
     U, V = cm.disc_state(p)
@@ -113,6 +113,31 @@ def run_batch(cm, steps, n=48, seed=20171026):
     return res
 
 
+def run_large(cm, np, steps):
+    """The large disc (768x768x384, ~226 M cells) with every source active:
+    power-law cooling Lambda ~ T^0.5 above T = 1e-2, spiral m=2 at 10 %,
+    GLM alpha = 0.1, MC + HLLD.  FP32 and FP64 fused paths on the same grid
+    (the plain pass-by-pass FP64 path does not fit one GPU's memory here),
+    with the FP32 stage kernel's HBM roofline fraction and the FP32 error
+    against FP64 after the same steps on the same dt sequence."""
+    import precision
+    grid = (768, 768, 384)
+    r32 = run_path(cm, np, cm.PATH_FUSED32, grid, steps, 1, config=5)
+    r64 = run_path(cm, np, cm.PATH_FUSED64, grid, steps, 1, config=5)
+    ncell = r32["interior_cells"]
+    bytes_launch = BYTES_PER_CELL_UPDATE["fused32"] * ncell / 3.0
+    achieved = bytes_launch / (r32["kernel_ms"] * 1e-3)
+    prec = precision.compare(cm.params_config(5, *grid), steps, cm.PATH_FUSED64, cm.PATH_FUSED32, log=log)
+    keep = ("l1_max", "physical", "divb_l1", "mass_rel", "energy_rel", "floor_hits")
+    return dict(grid=list(grid), interior_cells=ncell, steps=steps,
+                fused32={k: r32[k] for k in ("value", "ms_per_step", "kernel_ms")},
+                fused64={k: r64[k] for k in ("value", "ms_per_step", "kernel_ms")},
+                fp32_over_fp64=r32["value"] / r64["value"],
+                roofline=dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                              frac=achieved / HBM_PEAK, kernel="k_fused<float,MC,HLLD>"),
+                precision={k: prec[k] for k in keep if k in prec})
+
+
 def cpu_baseline(cm, np, budget_s=15.0):
     """CPU restatement of the reference on a bounded sample: the same disc at
     128x128x64, as many whole steps as fit the time budget (at least 1)."""
@@ -145,6 +170,9 @@ def main():
                     help="steps of the FP32-vs-FP64 error report (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=200,
                     help="steps of the 48-disc batching form (0: skip)")
+    ap.add_argument("--large-steps", type=int, default=10,
+                    help="steps of the 768x768x384 disc with every source active, FP32 and FP64 "
+                         "fused paths (0: skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device-to-device copy of this many MiB (the calibration "
                          "kernel of the FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -213,6 +241,11 @@ def main():
                                                  cm.PATH_PLAIN64, cm.PATH_FUSED32, log=log)
         except Exception as e:                     # reported, never fatal to the GPU number
             out["precision"] = {"error": str(e)}
+    if a.large_steps > 0:
+        try:
+            out["large"] = run_large(cm, np, a.large_steps)
+        except Exception as e:                     # reported, never fatal to the GPU number
+            out["large"] = {"error": str(e)}
     if a.batch_steps > 0:
         try:
             out["batch"] = run_batch(cm, a.batch_steps)
