@@ -24,6 +24,9 @@ struct FusedState {
     int dt_blocks = 0;
     bool dt_valid = false;
     int nseg = 1, seglen = 1, tiles_j = 1, tiles_k = 1;
+    /* r-segment counts the first stage-1 launch times (the fastest is kept) */
+    int ncand = 0, cand[8] = {0};
+    bool tuned = false;
     /* padded layout of the state arrays: component stride NS, k rows of
      * NZP cells with k = 1 at offset 16 (64-byte aligned FP32 tile rows) */
     size_t NS = 0;
@@ -31,7 +34,7 @@ struct FusedState {
     /* geometry packed for the stage kernel (few base pointers):
      *   itab [NX][IT] FP64  x, xm, oneox, oneodx, xabs, spiral r factors
      *   ikg  [NX*NZ][4] T   g1, g2, g3, g5
-     *   ikd  [NX*NZ][4] FP64 minlength, Grxaxi, Grzaxi, spiral q
+     *   ikd  [NX*NZ][4] FP64 minlength (1/minlength in the FP32 build), Grxaxi, Grzaxi, spiral q
      *   ktab [NZ][KT] FP64  oneodz, z, spiral z factors, zabs */
     double *itab = nullptr, *ikd = nullptr, *ktab = nullptr;
     void *ikg = nullptr;

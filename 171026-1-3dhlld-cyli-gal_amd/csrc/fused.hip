@@ -257,7 +257,24 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     }
     fs.seglen = (planes + nseg - 1) / nseg;
     fs.nseg = (planes + fs.seglen - 1) / fs.seglen;
-    fs.dt_blocks = std::max((cols * fs.nseg + 7) / 8 * 8, 1024);
+    /* The model above assumes the blocks run in lockstep rounds; they do
+     * not quite, and neighbouring counts differ by a few per cent.  The
+     * first stage-1 launch times the model's count and these candidates
+     * and keeps the fastest (the segmentation does not change the results:
+     * a segment's prologue recomputes the carried r-states and flux with
+     * the same arithmetic). */
+    fs.ncand = 0;
+    fs.cand[fs.ncand++] = fs.nseg;
+    int maxseg = fs.nseg;
+    for (int ns : {4, 5, 6, 8, 12, 16, 24}) {
+        if (fs.ncand >= 8 || planes / ns < 8) continue;
+        const int sl = (planes + ns - 1) / ns, nsr = (planes + sl - 1) / sl;
+        bool dup = false;
+        for (int q = 0; q < fs.ncand; q++) dup = dup || fs.cand[q] == nsr;
+        if (!dup) { fs.cand[fs.ncand++] = nsr; maxseg = std::max(maxseg, nsr); }
+    }
+    fs.tuned = fs.ncand < 2;
+    fs.dt_blocks = std::max((cols * maxseg + 7) / 8 * 8, 1024);
     if ((rc = alloc(2 * (size_t)fs.dt_blocks * sizeof(double), &p))) return rc;
     fs.dtpart = (double *)p;
     const size_t nik = (size_t)g.NX * g.NZ;
@@ -334,6 +351,42 @@ int setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, co
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
+/* Time the stage kernel of a first stage (its inputs are not overwritten:
+ * Un -> A) at each candidate segment count and keep the fastest.  Runs once
+ * per context, inside the first step; the extra launches' floor counts are
+ * cleared afterwards. */
+template <typename T, bool MC, bool HLL>
+static void tune_segments(hipStream_t s, FusedState &fs, FArgs<T> a)
+{
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) { fs.tuned = true; return; }
+    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); fs.tuned = true; return; }
+    const int planes = a.g.NX - 2;
+    float best = 1e30f;
+    int pick = fs.nseg;
+    for (int q = 0; q < fs.ncand; q++) {
+        const int ns = fs.cand[q];
+        a.seglen = (planes + ns - 1) / ns;
+        a.nblocks = fs.tiles_j * fs.tiles_k * ((planes + a.seglen - 1) / a.seglen);
+        const int nb = (a.nblocks + 7) / 8 * 8;
+        hipLaunchKernelGGL((k_fused<T, MC, HLL>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        hipEventRecord(e0, s);
+        for (int r = 0; r < 2; r++) hipLaunchKernelGGL((k_fused<T, MC, HLL>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        hipEventRecord(e1, s);
+        float ms = 0.0f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
+            best = ms;
+            pick = ns;
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    fs.seglen = (planes + pick - 1) / pick;
+    fs.nseg = (planes + fs.seglen - 1) / fs.seglen;
+    fs.tuned = true;
+    hipMemsetAsync(a.counts, 0, 3 * sizeof(int), s);
+}
+
 template <typename T, bool MC, bool HLL>
 static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
                     const StageArgs &sa, hipEvent_t ev0, hipEvent_t ev1)
@@ -351,9 +404,14 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
     a.Uin = in;
     a.Uout = out;
     a.nblocks = fs.tiles_j * fs.tiles_k * fs.nseg;
-    const int nb = (a.nblocks + 7) / 8 * 8;
     const int ne = (g.NY - 2) * (g.NZ - 2);
     hipLaunchKernelGGL((k_edge<T, MC, HLL>), dim3((ne + 255) / 256), dim3(256), 0, s, a, (T *)fs.edge);
+    if (!fs.tuned && sa.stage == 1) {
+        tune_segments<T, MC, HLL>(s, fs, a);
+        a.seglen = fs.seglen;
+        a.nblocks = fs.tiles_j * fs.tiles_k * fs.nseg;
+    }
+    const int nb = (a.nblocks + 7) / 8 * 8;
     if (ev0) hipEventRecord(ev0, s);
     hipLaunchKernelGGL((k_fused<T, MC, HLL>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
     if (ev1) hipEventRecord(ev1, s);

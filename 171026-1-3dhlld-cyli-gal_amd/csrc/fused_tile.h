@@ -206,7 +206,8 @@ __global__ void k_pack(DevGeom g, double *itab, T *ikg, double *ikd, double *kta
     const size_t nik = (size_t)g.NX * g.NZ;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < nik; q += (size_t)gridDim.x * blockDim.x) {
         ikg[4 * q + 0] = T(g.g1[q]); ikg[4 * q + 1] = T(g.g2[q]); ikg[4 * q + 2] = T(g.g3[q]); ikg[4 * q + 3] = T(g.g5[q]);
-        ikd[4 * q + 0] = g.minlength[q]; ikd[4 * q + 1] = g.grxaxi[q]; ikd[4 * q + 2] = g.grzaxi[q]; ikd[4 * q + 3] = g.sp_q[q];
+        /* FP32 build: 1/minlength (its Setdt multiplies); FP64: minlength */
+        ikd[4 * q + 0] = sizeof(T) == 4 ? 1.0 / g.minlength[q] : g.minlength[q]; ikd[4 * q + 1] = g.grxaxi[q]; ikd[4 * q + 2] = g.grzaxi[q]; ikd[4 * q + 3] = g.sp_q[q];
         if (q < (size_t)g.NX) {
             const int i = (int)q;
             double *r = itab + (size_t)i * IT;
@@ -589,7 +590,16 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             const bool spiral = gravity_staged(g, i, it, s_ikd[sl][ck], s_kt[ck], s_sc[sl][cj], gx, gy, gz);
             sources(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
 #pragma unroll
-            for (int m = 0; m < 9; m++) { u[m] = uin[m]; un[m] = unn[m]; }
+            for (int m = 0; m < 9; m++) {
+                /* the empty asm keeps the (zero or loaded) Un value in its
+                 * own precision up to here: otherwise the compiler widens it
+                 * to FP64 at the load, in front of the update barrier, and
+                 * waits for the loads there */
+                T w = unn[m];
+                asm("" : "+v"(w));
+                u[m] = uin[m];
+                un[m] = w;
+            }
             u[8] = T(c.psicoef) * u[8];
             un[8] = T(c.psicoef) * un[8];             /* Un is zero in stages that do not read it */
             /* The RK combination is formed in FP64 and rounded once: with FP32
@@ -618,12 +628,14 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             if (a.last) {
                 const bool ax = i >= g.NX - 11, ac = i <= g.irin, az = !a.rk3 && gk >= g.NZ - 11;
                 if (ax || ac || az) {
+                    double ui[9];                 /* every load issued before the first use */
+#pragma unroll
+                    for (int m = 0; m < 9; m++) ui[m] = a.Uinit[m * N + ccr];
 #pragma unroll
                     for (int m = 0; m < 9; m++) {
-                        const T ui = T(a.Uinit[m * N + ccr]);
-                        if (ax) u[m] -= (T(it[IT_XABS]) * (u[m] - ui));
-                        if (ac) u[m] -= (T(s_cabs[sl][ck]) * (u[m] - ui));
-                        if (az) u[m] -= (T(s_kt[ck][4]) * (u[m] - ui));
+                        if (ax) u[m] -= (T(it[IT_XABS]) * (u[m] - T(ui[m])));
+                        if (ac) u[m] -= (T(s_cabs[sl][ck]) * (u[m] - T(ui[m])));
+                        if (az) u[m] -= (T(s_kt[ck][4]) * (u[m] - T(ui[m])));
                     }
                 }
             }
@@ -637,8 +649,17 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                 const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
                 const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
                 const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
-                const double t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / s_ikd[sl][ck][0];
-                const double tc = (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7)));
+                double t1, tc;
+                if constexpr (sizeof(T) == 4) {
+                    /* FP64 still, with the row's stored 1/minlength and the
+                     * cooling term only when a cooling function is on
+                     * (otherwise it is 0 / (1.5 p) = 0) */
+                    t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) * s_ikd[sl][ck][0];
+                    tc = c.cooling ? (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7))) : 0.0;
+                } else {
+                    t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / s_ikd[sl][ck][0];
+                    tc = (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7)));
+                }
                 const double tt = rmax(t1, tc);
                 const long long ix = (long long)ccr;
                 if ((tt > best || (tt == best && ix < bix)) && tt > 0.0) { best = tt; bix = ix; }
