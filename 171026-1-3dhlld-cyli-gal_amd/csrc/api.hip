@@ -92,6 +92,7 @@ Consts make_consts(const cylmhd_ctx *c, double dt, double ch)
     k.te0 = c->p.te0;
     k.cooling = c->p.cooling;
     k.cool_lambda0 = c->p.cool_lambda0; k.cool_tmin = c->p.cool_tmin; k.cool_index = c->p.cool_index;
+    k.source = c->p.source;
     return k;
 }
 
@@ -113,6 +114,10 @@ int validate(const cylmhd_params *p)
     if (p->limiter < CYLMHD_LIMITER_MINMOD || p->limiter > CYLMHD_LIMITER_SUPERBEE || p->solver < CYLMHD_SOLVER_HLLD ||
         p->solver > CYLMHD_SOLVER_HLL) {
         snprintf(g_errmsg, sizeof g_errmsg, "unknown limiter %d or solver %d", p->limiter, p->solver);
+        return CYLMHD_EINVAL;
+    }
+    if (p->source < CYLMHD_SOURCE_SPIRAL_COOLING || p->source > CYLMHD_SOURCE_NONE) {
+        snprintf(g_errmsg, sizeof g_errmsg, "unknown source %d", p->source);
         return CYLMHD_EINVAL;
     }
     if (p->path != CYLMHD_PATH_PLAIN64 && p->limiter > CYLMHD_LIMITER_MC) {

@@ -61,7 +61,7 @@ __global__ __launch_bounds__((tile_threads<float, S>()), (tile_waves<float, S>()
     for (int it = b; it < nitems; it += gridDim.x) {
         const int4 w = items[it];
         if (!bd[w.x].alive) continue;
-        fused_tile<float, MC, HLL, S>(descs[w.x], w.y, w.z, it);
+        fused_tile<float, MC, HLL, S, false>(descs[w.x], w.y, w.z, it);   /* default source only */
     }
 }
 
@@ -480,6 +480,10 @@ int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out)
             p[d].rk != p[0].rk || p[d].limiter > CYLMHD_LIMITER_MC) {
             set_errmsg("batch: every disc must use the FP32 fused path and the same limiter (minmod/MC), solver and RK");
             return CYLMHD_EINVAL;
+        }
+        if (p[d].source != CYLMHD_SOURCE_SPIRAL_COOLING) {
+            set_errmsg("batch: the batching form runs the default source (SpiralGravityCooling) only");
+            return CYLMHD_EUNSUP;
         }
     }
     cylmhd_batch *b = new cylmhd_batch();

@@ -40,10 +40,10 @@ struct DevGeom {
  * (spiralgravity.h:83-113), built from time-independent tables plus the
  * per-step sin/cos of the arm phase.  Expression order as the reference. */
 __device__ __forceinline__ bool gravity_cell(const DevGeom &g, int i, int j, int k, double &gx,
-                                             double &gy, double &gz)
+                                             double &gy, double &gz, bool axi_only = false)
 {
     const size_t ik = (size_t)i * g.NZ + k;
-    if (i < g.irin) {
+    if (i < g.irin || axi_only) {
         gx = g.grxaxi[ik];
         gy = 0.0;
         gz = g.grzaxi[ik];

@@ -32,6 +32,7 @@ struct Consts {          /* scalars of one stage, passed by value */
     double omegasp;
     double te0;
     int    cooling;
+    int    source;            /* CYLMHD_SOURCE_* (in cooling's alignment gap) */
     double cool_lambda0, cool_tmin, cool_index;
 };
 
@@ -419,14 +420,48 @@ __device__ __forceinline__ void riemann(const Consts &c, const T *Vl, const T *V
     else hlld<DIR>(c, Vl, Vr, geo, oneox, F);
 }
 
-/* ---- SpiralGravityCooling (source.h:26-134) for one cell ---------------
- * gx, gy, gz: Grx.. (spiral region) or Grxaxi, 0, Grzaxi (i < irin).      */
-template <typename T>
+/* ---- source terms of one cell -------------------------------------------
+ * SpiralGravityCooling (source.h:26-134, the default), AxiGravityCooling
+ * (source.h:143-200) and AxiGravity (source.h:207-261) share one form: gx,
+ * gy, gz are Grx.. in the spiral region, else Grxaxi, 0, Grzaxi (everywhere
+ * for the two axisymmetric ones, whose callers pass spiral = false); AxiGravity
+ * has no cooling term.  UniformGz (source.h:269-300) and NoSource
+ * (source.h:308-349) use none of the gravity arguments; NoSource leaves S8
+ * as it was, which is 0 in every run.  ALT = false compiles the default
+ * (SpiralGravityCooling) only, for kernels launched with it.               */
+template <typename T, bool ALT = true>
 __device__ __forceinline__ void sources(const Consts &c, const T *V, T x, T oneox, T gx, T gy, T gz,
                                         bool spiral, T *S)
 {
     const T V0 = V[0], V1 = V[1], V2 = V[2], V3 = V[3], V4 = V[4], V5 = V[5], V6 = V[6], V7 = V[7];
+    if (ALT && c.source == 3) {
+#pragma unroll
+        for (int m = 0; m < 9; m++) S[m] = T(0.0);
+        S[3] = -V0 * T(1.0);
+        S[7] = -V3 * V0 * T(1.0);
+        return;
+    }
+    if (ALT && c.source == 4) {
+#pragma unroll
+        for (int m = 0; m < 9; m++) S[m] = T(0.0);
+        S[1] = (V0 * V2 * V2 - V5 * V5 + (V7 + T(0.5) * (V4 * V4 + V5 * V5 + V6 * V6))) * oneox;
+        S[5] = (V5 * V1 - V2 * V4) * oneox;
+        return;
+    }
     const T om = T(c.omegasp);
+    if (ALT && c.source == 2) {
+        S[0] = T(0.0);
+        S[1] = ((V0 * V2 * V2 - V5 * V5 + (V7 + T(0.5) * (V4 * V4 + V5 * V5 + V6 * V6))) * oneox
+                + V0 * gx + V0 * om * om * x + (T(2.0) * V0 * V2 * om));
+        S[2] = x * (T(-2.0) * V0 * V1 * om);
+        S[3] = V0 * gz;
+        S[4] = T(0.0);
+        S[5] = (V5 * V1 - V2 * V4) * oneox;
+        S[6] = T(0.0);
+        S[7] = (V0 * (V1 * gx + V3 * gz) + V0 * x * V1 * om * om);
+        S[8] = T(0.0);
+        return;
+    }
     const T lam = cooling(c, V0, V7);
     S[0] = T(0.0);
     S[1] = ((V0 * V2 * V2 - V5 * V5 + (V7 + T(0.5) * (V4 * V4 + V5 * V5 + V6 * V6))) * oneox

@@ -231,9 +231,9 @@ __global__ void k_pack(DevGeom g, double *itab, T *ikg, double *ikd, double *kta
  * k row (szs, z, 1 + a, z^2 s^2 term), sc = (sin, cos) of the arm phase
  * at (i, j) */
 __device__ __forceinline__ bool gravity_staged(const DevGeom &g, int i, cdp it, const double *ikd, const double *kt,
-                                               const double *sc, double &gx, double &gy, double &gz)
+                                               const double *sc, double &gx, double &gy, double &gz, bool axi_only)
 {
-    if (i < g.irin) {
+    if (i < g.irin || axi_only) {
         gx = ikd[1];
         gy = 0.0;
         gz = ikd[2];
@@ -276,8 +276,9 @@ template <typename T, int SHAPE> constexpr int tile_waves() { return Tile<T, SHA
 
 /* One RK stage on the work item (tile, segment): the r-march over planes
  * ibeg..iend of a TJ x TK (phi x z) column.  slot indexes the Setdt
- * partial the item writes on the last stage. */
-template <typename T, bool MC, bool HLL, int SHAPE>
+ * partial the item writes on the last stage.  ALT: any source function
+ * (Consts::source); false compiles SpiralGravityCooling only. */
+template <typename T, bool MC, bool HLL, int SHAPE, bool ALT>
 __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg, int slot)
 {
     constexpr int TJ = Tile<T, SHAPE>::TJ, TK = Tile<T, SHAPE>::TK;
@@ -587,8 +588,9 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             T v[9], S[9], u[9], un[9];
             ld9(s0 + cbase, v);
             double gx, gy, gz;
-            const bool spiral = gravity_staged(g, i, it, s_ikd[sl][ck], s_kt[ck], s_sc[sl][cj], gx, gy, gz);
-            sources(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
+            const bool spiral = gravity_staged(g, i, it, s_ikd[sl][ck], s_kt[ck], s_sc[sl][cj], gx, gy, gz,
+                                               ALT && (c.source == 1 || c.source == 2));
+            sources<T, ALT>(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
 #pragma unroll
             for (int m = 0; m < 9; m++) {
                 /* the empty asm keeps the (zero or loaded) Un value in its

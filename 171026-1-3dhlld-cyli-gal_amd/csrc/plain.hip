@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_update(DevGeom g, Consts c, const doubl
     double v[9], S[9];
     load9(V, N, cc, v);
     double gx, gy, gz;
-    const bool spiral = gravity_cell(g, i, j, k, gx, gy, gz);
+    const bool spiral = gravity_cell(g, i, j, k, gx, gy, gz, c.source == 1 || c.source == 2);
     sources(c, v, g.x[i], g.oneox[i], gx, gy, gz, spiral, S);
     const double g1 = g.g1[ik], g2 = g.g2[ik], g3 = g.g3[ik], g4 = g3, g5 = g.g5[ik], g6 = g5;
     const size_t f = F3(i, j, k), fi = F3(i - 1, j, k), fj = F3(i, j - 1, k), fk = F3(i, j, k - 1);

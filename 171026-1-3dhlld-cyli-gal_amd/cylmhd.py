@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libcylmhd.so")
 LIMITER_MINMOD, LIMITER_MC, LIMITER_VANLEER, LIMITER_SUPERBEE = 0, 1, 2, 3
 SOLVER_HLLD, SOLVER_HLL = 0, 1
 COOL_NONE, COOL_RAYMOND, COOL_POWERLAW = 0, 1, 2
+SOURCE_SPIRAL_COOLING, SOURCE_AXI_COOLING, SOURCE_AXI, SOURCE_UNIFORM_GZ, SOURCE_NONE = 0, 1, 2, 3, 4
 GRAV_GALACTIC, GRAV_POINTMASS = 0, 1
 PATH_PLAIN64, PATH_FUSED64, PATH_FUSED32 = 0, 1, 2
 OK, EINVAL, ENOMEM, EHIP, EIO, EUNSUP = 0, -1, -2, -3, -4, -5
@@ -43,7 +44,7 @@ class Params(C.Structure):
         ("armnum", C.c_int),
         ("omegasp", C.c_double), ("omegapattern", C.c_double), ("pitch", C.c_double),
         ("limiter", C.c_int), ("solver", C.c_int), ("rk", C.c_int),
-        ("cooling", C.c_int), ("gravity", C.c_int),
+        ("cooling", C.c_int), ("gravity", C.c_int), ("source", C.c_int),
         ("gm_point", C.c_double),
         ("cool_lambda0", C.c_double), ("cool_tmin", C.c_double), ("cool_index", C.c_double),
         ("path", C.c_int),

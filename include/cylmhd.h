@@ -46,6 +46,14 @@ extern "C" {
 #define CYLMHD_COOL_NONE      0
 #define CYLMHD_COOL_RAYMOND   1   /* cooling.h:22-78 (int-truncated)         */
 #define CYLMHD_COOL_POWERLAW  2   /* Lambda = lambda0 * T^index for T > tmin */
+/* source terms: the stage functions call SpiralGravityCooling
+ * (firststep.h:85); the reference's alternatives are its other functions in
+ * source.h, chosen by swapping that call                                  */
+#define CYLMHD_SOURCE_SPIRAL_COOLING 0  /* source.h:26-134 (default)         */
+#define CYLMHD_SOURCE_AXI_COOLING    1  /* source.h:143-200: Grxaxi/Grzaxi everywhere, cooling */
+#define CYLMHD_SOURCE_AXI            2  /* source.h:207-261: the same, no cooling term */
+#define CYLMHD_SOURCE_UNIFORM_GZ     3  /* source.h:269-300: g_z = -1, no hoop terms */
+#define CYLMHD_SOURCE_NONE           4  /* source.h:308-349: hoop and induction terms only */
 /* axisymmetric gravity model                                              */
 #define CYLMHD_GRAV_GALACTIC  0   /* initialgravity.h:22-79                  */
 #define CYLMHD_GRAV_POINTMASS 1   /* -GM r_vec/|r|^3 (Keplerian discs)       */
@@ -77,6 +85,7 @@ typedef struct cylmhd_params {
     /* scheme selection */
     int    limiter, solver, rk;  /* rk = 3 (FirstStep..ThirdStep) or 2     */
     int    cooling, gravity;
+    int    source;               /* CYLMHD_SOURCE_*                        */
     double gm_point;             /* GM of CYLMHD_GRAV_POINTMASS            */
     double cool_lambda0, cool_tmin, cool_index;
     int    path;                 /* CYLMHD_PATH_*                          */
@@ -181,7 +190,8 @@ typedef struct cylmhd_batch cylmhd_batch;
  * annulus, spiral amplitude uniform in [0, 0.1], its own seed; minmod +
  * HLLD, RK3, FP32 path.  p must hold n entries. */
 int cylmhd_params_batch(cylmhd_params *p, int n, uint64_t seed);
-/* All discs must share limiter (minmod or MC), solver and RK order. */
+/* All discs must share limiter (minmod or MC), solver and RK order, and use
+ * the default source (CYLMHD_SOURCE_SPIRAL_COOLING; others: CYLMHD_EUNSUP). */
 int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out);
 int cylmhd_batch_destroy(cylmhd_batch *b);
 /* The context of disc d (owned by the batch; do not destroy it). */

@@ -31,10 +31,14 @@ GRIDS = {
     "disc16": (16, 16, 10, "disc"),
     "rand16": (16, 8, 10, "rand"),
 }
-# scheme name -> (limiter define or None, HLL instead of HLLD)
-SCHEMES = {"mm_hlld": (None, False), "mc_hlld": ("REF_LIMITER_MC", False),
-           "mm_hll": (None, True), "mc_hll": ("REF_LIMITER_MC", True),
-           "vl_hlld": ("REF_LIMITER_VL", False), "sb_hlld": ("REF_LIMITER_SB", False)}
+# scheme name -> (limiter define or None, HLL instead of HLLD, source define or None)
+SCHEMES = {"mm_hlld": (None, False, None), "mc_hlld": ("REF_LIMITER_MC", False, None),
+           "mm_hll": (None, True, None), "mc_hll": ("REF_LIMITER_MC", True, None),
+           "vl_hlld": ("REF_LIMITER_VL", False, None), "sb_hlld": ("REF_LIMITER_SB", False, None),
+           "mc_hlld_axic": ("REF_LIMITER_MC", False, "REF_SOURCE_AXIC"),
+           "mc_hlld_axi": ("REF_LIMITER_MC", False, "REF_SOURCE_AXI"),
+           "mc_hlld_ugz": ("REF_LIMITER_MC", False, "REF_SOURCE_UGZ"),
+           "mc_hlld_none": ("REF_LIMITER_MC", False, "REF_SOURCE_NONE")}
 
 
 def hexf(v):
@@ -117,7 +121,7 @@ def main():
     src = os.path.join(HERE, "ref_harness.c")
     jobs = []
     for gname, (nr, nphi, nz, kind) in GRIDS.items():
-        for sname, (limdef, hll) in SCHEMES.items():
+        for sname, (limdef, hll, srcdef) in SCHEMES.items():
             name = f"{gname}_{sname}"
             d = os.path.join(OUT, name)
             os.makedirs(d, exist_ok=True)
@@ -129,6 +133,8 @@ def main():
                 cmd.insert(1, "-D" + limdef)
             if hll:
                 cmd.insert(1, "-DREF_SOLVER_HLL")
+            if srcdef:
+                cmd.insert(1, "-D" + srcdef)
             jobs.append((name, subprocess.Popen(cmd, stdout=subprocess.PIPE,
                                                 stderr=subprocess.STDOUT, text=True)))
     rc = 0

@@ -881,7 +881,11 @@ static void riemann(orc *o, int dir)
             }
 }
 
-/* SpiralGravityCooling (source.h:26-134) */
+/* Source terms, by p->source: SpiralGravityCooling (source.h:26-134, the
+ * default), AxiGravityCooling (source.h:143-200: Grxaxi/Grzaxi at every i),
+ * AxiGravity (source.h:207-261: the same without the cooling term),
+ * UniformGz (source.h:269-300) and NoSource (source.h:308-349, which leaves
+ * S8 untouched: 0 from the start) */
 static void sources(orc *o)
 {
     const cylmhd_params *p = &o->p;
@@ -889,10 +893,31 @@ static void sources(orc *o)
     const double *V = o->V;
     double *S = o->S;
     const double omegasp = p->omegasp;
+    const int mode = p->source;
     int i;
+    if (mode == CYLMHD_SOURCE_UNIFORM_GZ || mode == CYLMHD_SOURCE_NONE) {
+#pragma omp parallel for
+        for (i = 0; i < o->NX; i++)
+            for (int j = 0; j < o->NY; j++)
+                for (int k = 0; k < o->NZ; k++) {
+                    const size_t c = C3(o, i, j, k);
+                    const double V0 = V[c], V1 = V[N + c], V2 = V[2 * N + c], V3 = V[3 * N + c];
+                    const double V4 = V[4 * N + c], V5 = V[5 * N + c], V6 = V[6 * N + c], V7 = V[7 * N + c];
+                    for (int m = 0; m < 8; m++) S[m * N + c] = 0.0;
+                    if (mode == CYLMHD_SOURCE_UNIFORM_GZ) {
+                        S[3 * N + c] = -V0 * 1.0;
+                        S[7 * N + c] = -V3 * V0 * 1.0;
+                        S[8 * N + c] = 0.0;
+                    } else {
+                        S[1 * N + c] = (V0 * V2 * V2 - V5 * V5 + (V7 + 0.5 * (V4 * V4 + V5 * V5 + V6 * V6))) * o->oneox[i];
+                        S[5 * N + c] = (V5 * V1 - V2 * V4) * o->oneox[i];
+                    }
+                }
+        return;
+    }
 #pragma omp parallel for
     for (i = 0; i < o->NX; i++) {
-        const int spiral = i >= o->irin;
+        const int spiral = i >= o->irin && mode == CYLMHD_SOURCE_SPIRAL_COOLING;
         for (int j = 0; j < o->NY; j++)
             for (int k = 0; k < o->NZ; k++) {
                 const size_t c = C3(o, i, j, k);
@@ -913,6 +938,9 @@ static void sources(orc *o)
                 if (spiral)
                     S[7 * N + c] = (V0 * (V1 * gx + V2 * o->Gry[c] + V3 * gz)
                                     - (V0 * V0 * orc_cooling(p, V0, V7))
+                                    + V0 * x * V1 * omegasp * omegasp);
+                else if (mode == CYLMHD_SOURCE_AXI)
+                    S[7 * N + c] = (V0 * (V1 * gx + V3 * gz)
                                     + V0 * x * V1 * omegasp * omegasp);
                 else
                     S[7 * N + c] = (V0 * (V1 * gx + V3 * gz)

@@ -91,6 +91,18 @@
 #define MinmodLimiterY SuperbeeLimiterY
 #define MinmodLimiterZ SuperbeeLimiterZ
 #endif
+#ifdef REF_SOURCE_AXIC
+#define SpiralGravityCooling AxiGravityCooling
+#endif
+#ifdef REF_SOURCE_AXI
+#define SpiralGravityCooling AxiGravity
+#endif
+#ifdef REF_SOURCE_UGZ
+#define SpiralGravityCooling UniformGz
+#endif
+#ifdef REF_SOURCE_NONE
+#define SpiralGravityCooling NoSource
+#endif
 #ifdef REF_SOLVER_HLL
 #define HLLDX HLLX
 #define HLLDY HLLY

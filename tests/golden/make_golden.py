@@ -32,6 +32,15 @@ CASES = {
                                                         limiter=cm.LIMITER_MC, solver=cm.SOLVER_HLL, rk=2), 4, False),
     "rand16_mc_hlld_rk3": ("rand16_mc_hlld", "rand", dict(limiter=cm.LIMITER_MC, solver=cm.SOLVER_HLLD), 2, True),
     "rand16_mm_hll_rk2": ("rand16_mm_hll", "rand", dict(limiter=cm.LIMITER_MINMOD, solver=cm.SOLVER_HLL, rk=2), 2, True),
+    # the reference's other source functions (source.h:143-349), cooling band patch on
+    "rand16_mc_hlld_axic_rk3": ("rand16_mc_hlld_axic", "rand", dict(limiter=cm.LIMITER_MC, solver=cm.SOLVER_HLLD,
+                                                                  cooling_band=True, source=cm.SOURCE_AXI_COOLING), 2, True),
+    "rand16_mc_hlld_axi_rk3": ("rand16_mc_hlld_axi", "rand", dict(limiter=cm.LIMITER_MC, solver=cm.SOLVER_HLLD,
+                                                                cooling_band=True, source=cm.SOURCE_AXI), 2, True),
+    "rand16_mc_hlld_ugz_rk3": ("rand16_mc_hlld_ugz", "rand", dict(limiter=cm.LIMITER_MC, solver=cm.SOLVER_HLLD,
+                                                                cooling_band=True, source=cm.SOURCE_UNIFORM_GZ), 2, True),
+    "rand16_mc_hlld_none_rk3": ("rand16_mc_hlld_none", "rand", dict(limiter=cm.LIMITER_MC, solver=cm.SOLVER_HLLD,
+                                                                  cooling_band=True, source=cm.SOURCE_NONE), 2, True),
 }
 
 

@@ -91,6 +91,14 @@ def test_batch_tuner_keeps_the_states(gpu):
             assert e.max() <= 1e-6, (d, e)
 
 
+def test_batch_rejects_other_sources(gpu):
+    ps = batch_params()
+    ps[2].source = cm.SOURCE_AXI
+    with pytest.raises(cm.CylmhdError) as e:
+        cm.Batch(ps)
+    assert e.value.code == cm.EUNSUP
+
+
 def test_batch_stops_only_the_disc_below_dtmin(gpu):
     ps = batch_params(dtmin=1.0)          # disc 1: every dt is below dtmin
     with start_batch(ps) as b:

@@ -109,6 +109,42 @@ def test_fused32_disc(gpu):
         assert e.max() <= 1.0e-5, e
 
 
+SOURCES = [cm.SOURCE_AXI_COOLING, cm.SOURCE_AXI, cm.SOURCE_UNIFORM_GZ, cm.SOURCE_NONE]
+
+
+@pytest.mark.parametrize("source", SOURCES)
+def test_plain64_source_variants(gpu, source):
+    """The reference's other source functions (source.h:143-349) on the
+    random state with a patch in the cooling band: bit-level agreement as for
+    the default source (the oracle matches the reference on these cases)."""
+    p, Ui, Vi, Uc, Vc = rand_case(limiter=cm.LIMITER_MC, cooling_band=True, source=source)
+    hist = run_pair(p, Ui, Vi, Uc, Vc, 3, cm.PATH_PLAIN64)
+    for n, h in enumerate(hist):
+        print(f"step {n}: eU={h['eU'].max():.2e}")
+        assert h["eU"].max() <= TOL_STEP, h["eU"]
+        np.testing.assert_array_equal(h["cg"], h["co"])
+
+
+@pytest.mark.parametrize("source", SOURCES)
+def test_fused64_source_variants(gpu, source):
+    p, Ui, Vi, Uc, Vc = rand_case(limiter=cm.LIMITER_MC, consistent=True, nr=40, nphi=32, nz=34, source=source)
+    hist = run_pair(p, Ui, Vi, Uc, Vc, 1, cm.PATH_FUSED64)
+    h = hist[0]
+    print(f"eU={h['eU']}")
+    assert h["eU"].max() <= TOL_STEP, h["eU"]
+
+
+@pytest.mark.parametrize("source", SOURCES)
+def test_fused32_source_variants(gpu, source):
+    p, Ui, Vi, Uc, Vc = disc_case(3, 32, 32, 20, source=source)
+    x = cm.grid(p)[0]
+    hist = run_pair(p, Ui, Vi, Uc, Vc, 3, cm.PATH_FUSED32)
+    for n, h in enumerate(hist):
+        e = l1_rel_scaled(h["Ug"], h["Uo"], x)
+        print(f"step {n}: L1 rel {e}")
+        assert e.max() <= 1.0e-5, e
+
+
 @pytest.mark.parametrize("path", [cm.PATH_PLAIN64, cm.PATH_FUSED64])
 def test_totals_100_steps(gpu, path):
     """Total mass, energy and L1 div B tracked to 1e-10 relative over 100

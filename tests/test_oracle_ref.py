@@ -72,6 +72,36 @@ def test_random_state_matches_reference(scheme, rk):
     assert (counts.sum(axis=0) > 0).all(), counts    # all three floors fired
 
 
+SOURCES = {"axic": cm.SOURCE_AXI_COOLING, "axi": cm.SOURCE_AXI, "ugz": cm.SOURCE_UNIFORM_GZ,
+           "none": cm.SOURCE_NONE}
+
+
+@pytest.mark.parametrize("src", list(SOURCES))
+def test_source_variants_match_reference(src):
+    """The reference's other source functions (source.h:143-349), selected in
+    the harness the way its author swaps the SpiralGravityCooling call: the
+    random state with a patch in the Raymond cooling band, so the two
+    axisymmetric forms differ by their cooling term."""
+    p, Ui, Vi, Uc, Vc = rand_case(limiter=cm.LIMITER_MC, solver=cm.SOLVER_HLLD, cooling_band=True,
+                                  source=SOURCES[src])
+    compare(f"rand16_mc_hlld_{src}", p, Ui, Vi, Uc, Vc, 3, galactic=True)
+
+
+@pytest.mark.parametrize("src", list(SOURCES))
+def test_source_variants_differ(src):
+    """Each variant changes the state against SpiralGravityCooling with a 5 %
+    spiral at full amplitude after the first step (so the selection reaches
+    the stage update) and against every other variant."""
+    outs = {}
+    for name, mode in [("spiral", cm.SOURCE_SPIRAL_COOLING)] + list(SOURCES.items()):
+        p, Ui, Vi, Uc, Vc = rand_case(limiter=cm.LIMITER_MC, solver=cm.SOLVER_HLLD, cooling_band=True, source=mode,
+                                      epssp0=0.05, tausp=1.0e-30)
+        outs[name] = run_oracle(p, Ui, Vi, Uc, Vc, 2)[1]     # the spiral ramp is 0 at t = 0
+    for other, U in outs.items():
+        if other != src:
+            assert not np.array_equal(outs[src], U), (src, other)
+
+
 @pytest.mark.parametrize("scheme", ["mm_hlld", "mc_hlld"])
 def test_cooling_band_matches_reference(scheme):
     lim, sol = SCHEMES[scheme]
