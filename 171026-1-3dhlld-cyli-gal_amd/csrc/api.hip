@@ -16,6 +16,7 @@
 #include "batch.h"
 #include "fused.h"
 #include "paths.h"
+#include "textio.h"
 
 using namespace cyl;
 
@@ -46,6 +47,9 @@ struct cylmhd_ctx {
     FusedState fs{};
     std::vector<void *> allocs;
     double *d_sincos = nullptr, *d_logxs = nullptr;
+    double *d_dzm = nullptr;        /* [NZ] cell heights dzm (the column integral of a slice) */
+    double *d_slice = nullptr;      /* one slice, the largest plane */
+    size_t slice_cap = 0;
     bool have_state = false;
     /* timing of the per-stage hot kernels */
     bool timing = false;
@@ -173,6 +177,9 @@ int build_tables(cylmhd_ctx *c)
     const double *lx;
     if ((rc = dupload(c, &lx, c->sp.logxs, NX))) return rc;
     c->d_logxs = (double *)lx;
+    const double *dz;
+    if ((rc = dupload(c, &dz, g.dzm, NZ))) return rc;
+    c->d_dzm = (double *)dz;
     if ((rc = dalloc(c, &c->d_sincos, 2 * (size_t)NX * NY))) return rc;
     HIPCHK(hipMemset(c->d_sincos, 0, 2 * sizeof(double) * NX * NY));
     d.sincos = c->d_sincos;
@@ -206,6 +213,11 @@ int alloc_state(cylmhd_ctx *c)
         return rc;
     d.nred = 8192;
     if ((rc = dalloc(c, &d.red, (size_t)d.nred))) return rc;
+    {
+        const size_t rphi = (size_t)(NX - 2) * (NY - 2), xz = 2 * (size_t)NX * (NZ - 2);
+        c->slice_cap = rphi > xz ? rphi : xz;
+        if ((rc = dalloc(c, &c->d_slice, c->slice_cap))) return rc;
+    }
     HIPCHK(hipMemset(d.counts, 0, 9 * sizeof(int)));
     if (c->p.path == CYLMHD_PATH_PLAIN64) {
         if ((rc = dalloc(c, &d.U, 9 * d.N)) || (rc = dalloc(c, &d.U1, 9 * d.N)) || (rc = dalloc(c, &d.V, 9 * d.N)) ||
@@ -492,6 +504,192 @@ int cylmhd_kernel_timing(cylmhd_ctx *c, int which, int reset, double *avg_ms, in
     if (launches) *launches = c->ev_used;
     if (reset) c->ev_used = 0;
     c->timing = true;
+    return CYLMHD_OK;
+}
+
+}  // extern "C"
+
+/* ---- slices and text writers ------------------------------------------ */
+
+namespace {
+
+long long slice_count(const cylmhd_ctx *c, int plane)
+{
+    const long long NX = c->g.NX, NY = c->g.NY, NZ = c->g.NZ;
+    switch (plane) {
+    case CYLMHD_PLANE_RPHI:
+    case CYLMHD_PLANE_RPHI_ZMEAN:
+    case CYLMHD_PLANE_RPHI_ZINT: return (NX - 2) * (NY - 2);
+    case CYLMHD_PLANE_RZ:
+    case CYLMHD_PLANE_ZR: return (NX - 2) * (NZ - 2);
+    case CYLMHD_PLANE_XZ:
+    case CYLMHD_PLANE_XZ_PHIMEAN: return 2 * NX * (NZ - 2);
+    case CYLMHD_PLANE_PHI: return NY - 2;
+    }
+    return -1;
+}
+
+int slice_check(const cylmhd_ctx *c, int q, int plane, int a, int b)
+{
+    const int NX = c->g.NX, NY = c->g.NY, NZ = c->g.NZ;
+    if (q < 0 || q >= CYLMHD_NQ) return CYLMHD_EINVAL;
+    switch (plane) {
+    case CYLMHD_PLANE_RPHI: return a >= 0 && a < NZ ? CYLMHD_OK : CYLMHD_EINVAL;
+    case CYLMHD_PLANE_RPHI_ZMEAN:
+    case CYLMHD_PLANE_RPHI_ZINT:
+    case CYLMHD_PLANE_XZ_PHIMEAN: return CYLMHD_OK;
+    case CYLMHD_PLANE_RZ:
+    case CYLMHD_PLANE_ZR: return a >= 0 && a < NY ? CYLMHD_OK : CYLMHD_EINVAL;
+    case CYLMHD_PLANE_XZ: return a >= 1 && a <= NY - 2 && (NY - 2) % 2 == 0 ? CYLMHD_OK : CYLMHD_EINVAL;
+    case CYLMHD_PLANE_PHI: return a >= 0 && a < NX && b >= 0 && b < NZ ? CYLMHD_OK : CYLMHD_EINVAL;
+    }
+    return CYLMHD_EINVAL;
+}
+
+/* the FP64 primitives of the current state in ds.V (the fused paths keep
+ * conserved values only and convert here, as cylmhd_download_state does) */
+int primitives(cylmhd_ctx *c)
+{
+    if (c->p.path == CYLMHD_PATH_PLAIN64) return CYLMHD_OK;
+    Consts k = make_consts(c, 0.0, 0.0);
+    return fused_export(c->stream, c->fs, c->dg, c->ds, k, c->ds.V);
+}
+
+int slice_host(cylmhd_ctx *c, int q, int plane, int a, int b, double *out, size_t n)
+{
+    Consts k = make_consts(c, 0.0, 0.0);
+    launch_slice(c->stream, c->dg, k, c->ds.V, c->d_dzm, q, plane, a, b, c->d_slice, (int)n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, c->d_slice, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return CYLMHD_OK;
+}
+
+/* cut one slice (ds.V already current) and write it as text */
+int slice_file(cylmhd_ctx *c, std::vector<double> &buf, const char *dir, const char *name, int q, int plane,
+               int a)
+{
+    int rc = slice_check(c, q, plane, a, 0);
+    if (rc) return rc;
+    const size_t n = (size_t)slice_count(c, plane);
+    buf.resize(n);
+    if ((rc = slice_host(c, q, plane, a, 0, buf.data(), n))) return rc;
+    return cyl_write_text(dir, name, buf.data(), n);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cylmhd_slice_size(const cylmhd_ctx *c, int plane, size_t *n)
+{
+    if (!c || !n) return CYLMHD_EINVAL;
+    const long long m = slice_count(c, plane);
+    if (m < 0) return CYLMHD_EINVAL;
+    *n = (size_t)m;
+    return CYLMHD_OK;
+}
+
+int cylmhd_slice(cylmhd_ctx *c, int quantity, int plane, int a, int b, double *out, size_t n)
+{
+    if (!c || !out || !c->have_state) return CYLMHD_EINVAL;
+    int rc = slice_check(c, quantity, plane, a, b);
+    if (rc) return rc;
+    if ((long long)n != slice_count(c, plane)) return CYLMHD_EINVAL;
+    if ((rc = primitives(c))) return rc;
+    return slice_host(c, quantity, plane, a, b, out, n);
+}
+
+int cylmhd_gdl_output_ctx(cylmhd_ctx *c, const char *dir, int nout)
+{
+    /* GDLOutputCylinder (output.h:280-384) with the planes cut on the device */
+    if (!c || !dir || !c->have_state) return CYLMHD_EINVAL;
+    int rc;
+    if ((rc = cyl_mkdir(dir))) return rc;
+    const cyl_grid &g = c->g;
+    if ((rc = cyl_write_grid_text(dir, "r.dat", "phi.dat", "z.dat", g.NX, g.NY, g.NZ, g.x, g.y, g.z))) return rc;
+    if ((rc = primitives(c))) return rc;
+    std::vector<double> buf;
+    char name[128];
+    static const char *what[3] = {"density", "pressure", "benergy"};
+    static const int qs[3] = {0, 7, CYLMHD_Q_BENERGY};
+    for (int w = 0; w < 3; w++) {
+        snprintf(name, sizeof name, "2d-%s-rphi-%d.dat", what[w], nout);
+        if ((rc = slice_file(c, buf, dir, name, qs[w], CYLMHD_PLANE_RPHI, 1))) return rc;
+    }
+    snprintf(name, sizeof name, "2d-density-rz-%d.dat", nout);
+    return slice_file(c, buf, dir, name, 0, CYLMHD_PLANE_RZ, 0);
+}
+
+int cylmhd_idl_output(cylmhd_ctx *c, const char *root, int nout, const cylmhd_idl_planes *planes)
+{
+    if (!c || !root || !c->have_state || nout < 0) return CYLMHD_EINVAL;
+    const cyl_grid &g = c->g;
+    const int NX = g.NX, NY = g.NY, NZ = g.NZ;
+    cylmhd_idl_planes o = {1, 100, 1};
+    if (planes) o = *planes;
+    if (o.k100 > NZ - 2) o.k100 = NZ - 2;
+    if (o.kplane < 1 || o.kplane > NZ - 2 || o.k100 < 1 || o.jplane < 1 || o.jplane > NY - 2 || (NY - 2) % 2)
+        return CYLMHD_EINVAL;
+    char dir0[4096], dir[4096];
+    if (snprintf(dir0, sizeof dir0, "%s/idl0", root) >= (int)sizeof dir0 ||
+        snprintf(dir, sizeof dir, "%s/idl%d", root, nout) >= (int)sizeof dir)
+        return CYLMHD_EINVAL;
+    int rc;
+    if ((rc = cyl_mkdir(root)) || (rc = cyl_mkdir(dir0)) || (rc = cyl_mkdir(dir))) return rc;
+    /* coordinates: r, phi, z of the interior cells; x of the cut through
+     * the axis, -x[NX-1] .. -x[0], x[0] .. x[NX-1] (2*NX values) */
+    std::vector<double> xs(2 * (size_t)NX);
+    for (int i = 0; i < NX; i++) {
+        xs[(size_t)(NX - 1 - i)] = -g.x[i];
+        xs[(size_t)(NX + i)] = g.x[i];
+    }
+    for (const char *d : {(const char *)dir0, (const char *)dir}) {
+        if ((rc = cyl_write_grid_text(d, "r.txt", "phi.txt", "z.txt", NX, NY, NZ, g.x, g.y, g.z)) ||
+            (rc = cyl_write_text(d, "x.txt", xs.data(), xs.size())))
+            return rc;
+        if (nout == 0) break;
+    }
+    if ((rc = primitives(c))) return rc;
+    struct Item {
+        const char *name;
+        int q, plane, a;
+    };
+    const int kp = o.kplane, jp = o.jplane;
+    const Item items[] = {
+        /* r-phi planes, fltarr(ny, nx): density.pro, beta.pro, benergy.pro, bphi0.pro, bphi100.pro,
+         * br0.pro, divb0.pro, faradaydepth.pro, vdotb.pro, temperature.pro */
+        {"zplanerho", 0, CYLMHD_PLANE_RPHI, kp},
+        {"zplanebeta", CYLMHD_Q_BETA, CYLMHD_PLANE_RPHI, kp},
+        {"zplanebenergy", CYLMHD_Q_BENERGY, CYLMHD_PLANE_RPHI, kp},
+        {"z0bphi", 5, CYLMHD_PLANE_RPHI, 1},
+        {"z100bphi", 5, CYLMHD_PLANE_RPHI, o.k100},
+        {"zplanemeanbr", 4, CYLMHD_PLANE_RPHI_ZMEAN, 0},
+        {"divb0", CYLMHD_Q_DIVB, CYLMHD_PLANE_RPHI, kp},
+        {"faradaydepth-rphi", CYLMHD_Q_RHOBZ, CYLMHD_PLANE_RPHI_ZINT, 0},
+        {"vdotb", CYLMHD_Q_VDOTB, CYLMHD_PLANE_RPHI, kp},
+        {"temp-rphi", CYLMHD_Q_TEMP, CYLMHD_PLANE_RPHI, kp},
+        /* r-z planes, fltarr(nz, nx): density-rz.pro, vphi-rz.pro, bphi-rz.pro, temp-rz.pro */
+        {"rho-rz", 0, CYLMHD_PLANE_ZR, jp},
+        {"vphi-rz", 2, CYLMHD_PLANE_ZR, jp},
+        {"bphi-rz", 5, CYLMHD_PLANE_ZR, jp},
+        {"temp-rz", CYLMHD_Q_TEMP, CYLMHD_PLANE_ZR, jp},
+        /* cuts through the axis, fltarr(2*NX, nz): density-xz.pro, beta-xz.pro (and tcool-xz.pro),
+         * bphi-xz.pro, divb-xz.pro, temp-xz.pro, vz-xz.pro, oneotcool-xz.pro */
+        {"rho-xz", 0, CYLMHD_PLANE_XZ, jp},
+        {"xzbeta", CYLMHD_Q_BETA, CYLMHD_PLANE_XZ, jp},
+        {"xzmeanbphi", 5, CYLMHD_PLANE_XZ_PHIMEAN, 0},
+        {"divb-xz", CYLMHD_Q_DIVB, CYLMHD_PLANE_XZ, jp},
+        {"temp-xz", CYLMHD_Q_TEMP, CYLMHD_PLANE_XZ, jp},
+        {"xzvz", 3, CYLMHD_PLANE_XZ, jp},
+        {"oneocoolingtime-xz", CYLMHD_Q_ONEOTCOOL, CYLMHD_PLANE_XZ, jp},
+    };
+    std::vector<double> buf;
+    char name[160];
+    for (const Item &it : items) {
+        snprintf(name, sizeof name, "%s-%d.dat", it.name, nout);
+        if ((rc = slice_file(c, buf, dir, name, it.q, it.plane, it.a))) return rc;
+    }
     return CYLMHD_OK;
 }
 

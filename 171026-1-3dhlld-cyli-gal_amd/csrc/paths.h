@@ -40,6 +40,10 @@ void launch_sincos(hipStream_t s, const DevGeom &g, double *sincos, const double
                    double omegapattern, double omegasp);
 void launch_setdt(hipStream_t s, const DevGeom &g, const Consts &c, const double *V, double *red, int nred);
 void launch_diag(hipStream_t s, const DevGeom &g, const double *U, double *red, int nred);
+/* one slice of quantity q (CYLMHD_Q_* or 0..8) over plane (CYLMHD_PLANE_*)
+ * of the FP64 primitive state V into out[n] (slices.hip) */
+void launch_slice(hipStream_t s, const DevGeom &g, const Consts &c, const double *V, const double *dzm, int q,
+                  int plane, int a, int b, double *out, int n);
 
 }  // namespace cyl
 #endif

@@ -22,6 +22,12 @@ COOL_NONE, COOL_RAYMOND, COOL_POWERLAW = 0, 1, 2
 SOURCE_SPIRAL_COOLING, SOURCE_AXI_COOLING, SOURCE_AXI, SOURCE_UNIFORM_GZ, SOURCE_NONE = 0, 1, 2, 3, 4
 GRAV_GALACTIC, GRAV_POINTMASS = 0, 1
 PATH_PLAIN64, PATH_FUSED64, PATH_FUSED32 = 0, 1, 2
+# slice quantities beyond the primitives 0..8 and slice planes (include/cylmhd.h)
+Q_RHO, Q_VR, Q_VPHI, Q_VZ, Q_BR, Q_BPHI, Q_BZ, Q_PR, Q_PSI = range(9)
+Q_BENERGY, Q_BETA, Q_TEMP, Q_ONEOTCOOL, Q_VDOTB, Q_DIVB, Q_RHOBZ = range(9, 16)
+NQ = 16
+(PLANE_RPHI, PLANE_RPHI_ZMEAN, PLANE_RPHI_ZINT, PLANE_RZ, PLANE_ZR, PLANE_XZ,
+ PLANE_XZ_PHIMEAN, PLANE_PHI) = range(8)
 OK, EINVAL, ENOMEM, EHIP, EIO, EUNSUP = 0, -1, -2, -3, -4, -5
 PATH_NAMES = {PATH_PLAIN64: "plain64", PATH_FUSED64: "fused64", PATH_FUSED32: "fused32"}
 
@@ -68,6 +74,10 @@ class Params(C.Structure):
 class Diag(C.Structure):
     _fields_ = [("mass", C.c_double), ("energy", C.c_double),
                 ("divb_l1", C.c_double), ("psi_l1", C.c_double)]
+
+
+class IdlPlanes(C.Structure):
+    _fields_ = [("kplane", C.c_int), ("k100", C.c_int), ("jplane", C.c_int)]
 
 
 class CylmhdError(RuntimeError):
@@ -117,6 +127,11 @@ def lib():
         "cylmhd_binary_output": ([C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double,
                                   _dp, _dp, _dp, _dp], C.c_int),
         "cylmhd_restart_read": ([C.c_char_p, C.c_int, C.c_int, C.c_int, _dp], C.c_int),
+        "cylmhd_slice_size": ([C.c_void_p, C.c_int, C.POINTER(C.c_size_t)], C.c_int),
+        "cylmhd_slice": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, C.c_size_t], C.c_int),
+        "cylmhd_gdl_output": ([C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp], C.c_int),
+        "cylmhd_gdl_output_ctx": ([C.c_void_p, C.c_char_p, C.c_int], C.c_int),
+        "cylmhd_idl_output": ([C.c_void_p, C.c_char_p, C.c_int, C.POINTER(IdlPlanes)], C.c_int),
         "cylmhd_params_batch": ([P, C.c_int, C.c_uint64], C.c_int),
         "cylmhd_batch_create": ([P, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
         "cylmhd_batch_destroy": ([C.c_void_p], C.c_int),
@@ -209,6 +224,13 @@ def binary_output(dirname, t, x, y, z, V):
                                      np.ascontiguousarray(V)), "binary_output")
 
 
+def gdl_output(dirname, nout, x, y, z, V):
+    """GDLOutputCylinder's text files (output.h:280-384) from host arrays."""
+    nx, ny, nz = V.shape[1:]
+    check(lib().cylmhd_gdl_output(dirname.encode(), nout, nx, ny, nz, x, y, z,
+                                  np.ascontiguousarray(V)), "gdl_output")
+
+
 def restart_read(dirname, shape):
     V = np.zeros((9,) + tuple(shape))
     check(lib().cylmhd_restart_read(dirname.encode(), shape[0], shape[1], shape[2], V), "restart_read")
@@ -255,6 +277,22 @@ class Context:
         V = np.zeros((9,) + self.dims)
         check(lib().cylmhd_download_state(self.h, U, V), "download")
         return U, V
+
+    def slice(self, quantity, plane, a=0, b=0):
+        """One plane of a primitive or derived quantity, cut on the device."""
+        n = C.c_size_t()
+        check(lib().cylmhd_slice_size(self.h, plane, C.byref(n)), "slice_size")
+        out = np.zeros(n.value)
+        check(lib().cylmhd_slice(self.h, quantity, plane, a, b, out, n.value), "slice")
+        return out
+
+    def gdl_output(self, dirname, nout):
+        check(lib().cylmhd_gdl_output_ctx(self.h, dirname.encode(), nout), "gdl_output_ctx")
+
+    def idl_output(self, root, nout, kplane=1, k100=100, jplane=1):
+        """The text slices the reference's IDL scripts read (idl<nout>/...)."""
+        pl = IdlPlanes(kplane, k100, jplane)
+        check(lib().cylmhd_idl_output(self.h, root.encode(), nout, C.byref(pl)), "idl_output")
 
     def setdt(self):
         dt, ch = C.c_double(), C.c_double()

@@ -19,6 +19,7 @@
 #ifndef CYLMHD_H
 #define CYLMHD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -216,6 +217,57 @@ int cylmhd_binary_output(const char *dir, int nx, int ny, int nz, double t,
                          const double *V);
 /* RestartRead (restartread.h:9-143): reads the nine primitive files. */
 int cylmhd_restart_read(const char *dir, int nx, int ny, int nz, double *V);
+
+/* ---- slices of the resident state and the text writers ----------------
+ * The reference's text writer is GDLOutputCylinder (output.h:280-384); its
+ * 31 IDL scripts read text slices ("%e" one value a line) from ./idlN/,
+ * written by a converter that is not in the reference.  cylmhd_slice cuts
+ * a plane out of the state on the device (FP64 primitives; the fused paths
+ * convert first), so a slice never needs the whole state on the host. */
+/* quantity: 0..8 are the primitives V[m] (rho vr vphi vz br bphi bz pr psi) */
+#define CYLMHD_Q_BENERGY    9   /* 0.5 |B|^2 (output.h:350-360)                       */
+#define CYLMHD_Q_BETA      10   /* pr / (0.5 |B|^2): gas over magnetic pressure        */
+#define CYLMHD_Q_TEMP      11   /* pr / rho, the temperature Cooling uses (cooling.h:25) */
+#define CYLMHD_Q_ONEOTCOOL 12   /* 2 rho^2 Lambda / (1.5 pr) (setdt.h:56-58)            */
+#define CYLMHD_Q_VDOTB     13   /* v . B                                               */
+#define CYLMHD_Q_DIVB      14   /* cell div B as cylmhd_diagnostics (0 on ghost cells) */
+#define CYLMHD_Q_RHOBZ     15   /* rho Bz, the Faraday-depth integrand                 */
+#define CYLMHD_NQ          16
+/* plane (a, b are indices with the ghosts counted, as the reference's) */
+#define CYLMHD_PLANE_RPHI        0 /* k = a: i outer, j inner, interior i, j (output.h:318-330) */
+#define CYLMHD_PLANE_RPHI_ZMEAN  1 /* mean over the interior k, same order             */
+#define CYLMHD_PLANE_RPHI_ZINT   2 /* sum over the interior k of q dzm[k-1]            */
+#define CYLMHD_PLANE_RZ          3 /* j = a: k outer, i inner, interior (output.h:368-380) */
+#define CYLMHD_PLANE_ZR          4 /* j = a: i outer, k inner, interior                */
+#define CYLMHD_PLANE_XZ          5 /* cut through the axis: k outer (interior), 2*NX columns:
+                                      the opposite half-plane j' = a + (NY-2)/2 (mod the
+                                      interior) from i = NX-1 down to 0, then j = a from
+                                      i = 0 up; 1 <= a <= NY-2, NY-2 even              */
+#define CYLMHD_PLANE_XZ_PHIMEAN  6 /* as XZ with the mean over the interior j on both halves */
+#define CYLMHD_PLANE_PHI         7 /* i = a, k = b: the interior j                     */
+/* Number of values of a plane. */
+int cylmhd_slice_size(const cylmhd_ctx *c, int plane, size_t *n);
+/* One slice of the current state into host out[n] (n = cylmhd_slice_size). */
+int cylmhd_slice(cylmhd_ctx *c, int quantity, int plane, int a, int b, double *out, size_t n);
+
+/* GDLOutputCylinder (output.h:280-384) on host arrays: <dir>/r.dat, phi.dat,
+ * z.dat (interior coordinates) and 2d-density-rphi-<nout>.dat,
+ * 2d-pressure-rphi-, 2d-benergy-rphi- (k = 1), 2d-density-rz- (j = 0). */
+int cylmhd_gdl_output(const char *dir, int nout, int nx, int ny, int nz,
+                      const double *x, const double *y, const double *z, const double *V);
+/* The same files from a context, the planes cut on the device. */
+int cylmhd_gdl_output_ctx(cylmhd_ctx *c, const char *dir, int nout);
+
+typedef struct cylmhd_idl_planes {
+    int kplane;   /* k of the z-plane slices (zplane*, divb0, vdotb, temp-rphi); 1 */
+    int k100;     /* k of z100bphi; 100, clipped to NZ-2                              */
+    int jplane;   /* j of the rz cuts and of the xz cut through the axis; 1          */
+} cylmhd_idl_planes;
+/* The text slices the reference's IDL scripts read, for dump number nout:
+ * <root>/idl<nout>/ (and <root>/idl0/) r.txt, phi.txt, z.txt, x.txt, and
+ * <root>/idl<nout>/<name>-<nout>.dat for the names the scripts open, each
+ * in the order its fltarr + readf expects.  planes may be NULL (defaults). */
+int cylmhd_idl_output(cylmhd_ctx *c, const char *root, int nout, const cylmhd_idl_planes *planes);
 
 #ifdef __cplusplus
 }

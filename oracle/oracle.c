@@ -238,6 +238,12 @@ void orc_grid(const orc *o, double *x, double *xm, double *y, double *z, double 
     if (mml) *mml = o->minminlength;
 }
 
+/* dzm[k] = 0.5 (dz[k] + dz[k+1]) (cylindricalgrid.h:94): the height of cell k+1 */
+void orc_dzm(const orc *o, double *dzm)
+{
+    memcpy(dzm, o->dzm, sizeof(double) * o->NZ);
+}
+
 void orc_geometry(const orc *o, double *out)
 {
     const size_t T = (size_t)o->NX * o->NZ;

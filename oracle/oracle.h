@@ -30,6 +30,7 @@ void   orc_dims(const orc *o, int *nx, int *ny, int *nz);
 void   orc_grid(const orc *o, double *x, double *xm, double *y, double *z, double *minminlength);
 /* g-factor / volume tables: out[6][NX][NZ] = g1,g2,g3,g5,dvolume,minlength */
 void   orc_geometry(const orc *o, double *out);
+void   orc_dzm(const orc *o, double *dzm);
 /* absorbing profiles: xabs[11], zabs[11], cabs[(irin+1)*NZ]; returns irin */
 int    orc_absorb_factors(const orc *o, double *xabs, double *zabs, double *cabs);
 /* axisymmetric gravity [NX*NZ] tables actually used */

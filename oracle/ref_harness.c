@@ -23,7 +23,8 @@
  *        last stage; x,xm[NX]; y[NY]; z[NZ]; minminlength; g1,g2,g3,g5,
  *        dvolume,minlength [NX*NZ] (j = 1); grxaxi,grzaxi[NX*NZ];
  *        x_absorb_factor[11]; c_absorb_factor[(irin+1)*NZ]
- *   DUMPDIR: if given, BinaryOutput() of the final state is written there.
+ *   DUMPDIR: if given, BinaryOutput() of the final state is written there,
+ *            and GDLOutputCylinder()'s text files (nout = 7) into DUMPDIR.
  */
 #define _GNU_SOURCE
 #include <stdio.h>
@@ -258,6 +259,9 @@ int main(int argc, char **argv)
         if (chdir(argv[5]) != 0) { perror(argv[5]); return 2; }
         nout = 0;
         BinaryOutput();
+        /* the reference's text writer, into DUMPDIR itself, dump number 7 */
+        nout = 7;
+        GDLOutputCylinder();
     }
     (void)m;
     return 0;

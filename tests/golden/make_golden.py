@@ -14,6 +14,7 @@ import hashlib
 import json
 import os
 import sys
+import tempfile
 
 import numpy as np
 
@@ -44,6 +45,11 @@ CASES = {
 }
 
 
+# cases whose run also writes GDLOutputCylinder's text files (output.h:280-384,
+# dump number 7): stored as bytes in golden/text/<name>_gdl.npz
+TEXT_CASES = ("disc16_mm_hlld_rk3",)
+
+
 def build_inputs(kind, kw):
     if kind == "disc":
         kw = dict(kw)
@@ -67,7 +73,16 @@ def main():
         p, Ui, Vi, Uc, Vc = build_inputs(kind, kw)
         o = Oracle(p)
         gx, gz = o.gravity_axi()
-        r, out = run_ref(harness, Vi, Vc, gx, gz, nsteps, p.rk, galactic=galactic, U_init=Ui, U_cur=Uc)
+        with tempfile.TemporaryDirectory() as td:
+            dumpdir = td if name in TEXT_CASES else None
+            r, out = run_ref(harness, Vi, Vc, gx, gz, nsteps, p.rk, galactic=galactic, U_init=Ui, U_cur=Uc,
+                             dumpdir=dumpdir)
+            if dumpdir:
+                texts = {f: np.frombuffer(open(os.path.join(td, f), "rb").read(), dtype=np.uint8)
+                         for f in sorted(os.listdir(td)) if os.path.isfile(os.path.join(td, f))}
+                os.makedirs(os.path.join(HERE, "text"), exist_ok=True)
+                np.savez_compressed(os.path.join(HERE, "text", name + "_gdl.npz"), **texts)
+                print(f"wrote text/{name}_gdl.npz: {sorted(texts)}")
         counts = parse_counts(out)
         nst = 3 if p.rk == 3 else 2
         np.savez_compressed(

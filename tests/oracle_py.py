@@ -33,6 +33,7 @@ def _load(params_cls):
         "orc_dims": ([vp, C.POINTER(C.c_int)] * 1 + [C.POINTER(C.c_int), C.POINTER(C.c_int)], None),
         "orc_grid": ([vp, _dp, _dp, _dp, _dp, C.POINTER(C.c_double)], None),
         "orc_geometry": ([vp, _dp], None),
+        "orc_dzm": ([vp, _dp], None),
         "orc_absorb_factors": ([vp, _dp, _dp, _dp], C.c_int),
         "orc_gravity_axi": ([vp, _dp, _dp], None),
         "orc_set_initial": ([vp, _dp, _dp], None),
@@ -95,6 +96,11 @@ class Oracle:
         NX, NY, NZ = self.dims
         out = np.zeros((6, NX, NZ))
         self.L.orc_geometry(self.h, out)
+        return out
+
+    def dzm(self):
+        out = np.zeros(self.dims[2])
+        self.L.orc_dzm(self.h, out)
         return out
 
     def absorb_factors(self):
