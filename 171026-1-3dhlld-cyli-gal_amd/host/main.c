@@ -11,9 +11,15 @@
  *
  *   cylmhd_run [--config N] [--grid NR NPHI NZ] [--path plain64|fused64|fused32]
  *              [--rk 2|3] [--nstop N] [--tint T] [--tstop T] [--restart NOUT]
- *              [--outdir DIR] [--quiet]
+ *              [--outdir DIR] [--quiet] [--gdl] [--idl ROOT] [--convert FIRST LAST]
  * --config 0 (default) is the reference's own set-up: input.h grid and
- * GalacticDisc3.  Exit status: 0, or 1 on the dt guard / any library error.
+ * GalacticDisc3.  With every dump: --gdl also writes GDLOutputCylinder's
+ * text files (output.h:280-384; main.c:371-372 has the call commented) into
+ * DIR, --idl the text slices the reference's IDL scripts read into
+ * ROOT/idl<nout>/.  --convert FIRST LAST runs no step: it reads the dumps
+ * DIR/FIRST .. DIR/LAST (their grid must be the one --config/--grid gives)
+ * and writes their text files.  Exit status: 0, or 1 on the dt guard / any
+ * library error.
  */
 #include <math.h>
 #include <stdio.h>
@@ -27,6 +33,22 @@ static int fail(const char *what, int rc)
 {
     fprintf(stderr, "%s: %s\n", what, cylmhd_error_string(rc));
     return 1;
+}
+
+/* the raw coordinate file <dir>/<name> of a dump equals v[n] */
+static int same_coords(const char *dir, const char *name, const double *v, int n)
+{
+    char path[4200];
+    snprintf(path, sizeof path, "%s/%s", dir, name);
+    FILE *f = fopen(path, "rb");
+    if (!f) return 0;
+    int ok = 1;
+    for (int q = 0; q < n && ok; q++) {
+        double w;
+        ok = fread(&w, sizeof w, 1, f) == 1 && w == v[q];
+    }
+    fclose(f);
+    return ok;
 }
 
 static void dirname_of(char *buf, size_t n, const char *outdir, int nout)
@@ -53,9 +75,20 @@ static void galactic_u_bcs(int NX, int NY, int NZ, double *U)
             }
 }
 
+/* the text files of dump nout from the context's current state */
+static int text_output(cylmhd_ctx *ctx, int gdl, const char *idl, const char *outdir, int nout)
+{
+    int rc;
+    if (gdl && (rc = cylmhd_gdl_output_ctx(ctx, outdir, nout))) return fail("gdl_output", rc);
+    if (idl && (rc = cylmhd_idl_output(ctx, idl, nout, NULL))) return fail("idl_output", rc);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
-    int config = 0, nr = 0, nphi = 0, nz = 0, path = -1, rk = 0, restart = -1, quiet = 0;
+    int config = 0, nr = 0, nphi = 0, nz = 0, path = -1, rk = 0, restart = -1, quiet = 0, gdl = 0;
+    int conv_first = -1, conv_last = -1;
+    const char *idl = NULL;
     long nstop = 10000000;
     double tint = 0.05, tstop = 40.0;                      /* input.h:82-84 */
     const char *outdir = ".";
@@ -74,6 +107,9 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[a], "--restart") && a + 1 < argc) restart = atoi(argv[++a]);
         else if (!strcmp(argv[a], "--outdir") && a + 1 < argc) outdir = argv[++a];
         else if (!strcmp(argv[a], "--quiet")) quiet = 1;
+        else if (!strcmp(argv[a], "--gdl")) gdl = 1;
+        else if (!strcmp(argv[a], "--idl") && a + 1 < argc) idl = argv[++a];
+        else if (!strcmp(argv[a], "--convert") && a + 2 < argc) { conv_first = atoi(argv[++a]); conv_last = atoi(argv[++a]); }
         else { fprintf(stderr, "usage: see the header of host/main.c\n"); return 2; }
     }
     if (path == -2) { fprintf(stderr, "unknown --path\n"); return 2; }
@@ -97,6 +133,25 @@ int main(int argc, char **argv)
     char dir[4096];
     cylmhd_ctx *ctx = NULL;
     if ((rc = cylmhd_create(&p, &ctx))) return fail("create", rc);
+    if (conv_first >= 0) {
+        /* the dump -> text converter: no step */
+        if (!gdl && !idl) { fprintf(stderr, "--convert needs --gdl and/or --idl ROOT\n"); return 2; }
+        for (int n = conv_first; n <= conv_last; n++) {
+            dirname_of(dir, sizeof dir, outdir, n);
+            if (!same_coords(dir, "r", x, NX) || !same_coords(dir, "phi", y, NY) || !same_coords(dir, "z", z, NZ)) {
+                fprintf(stderr, "%s: missing, or its grid is not the one of --config/--grid\n", dir);
+                return 1;
+            }
+            if ((rc = cylmhd_restart_read(dir, NX, NY, NZ, V))) return fail(dir, rc);
+            if ((rc = cylmhd_vtou_host(&p, V, U))) return fail("vtou", rc);
+            if ((rc = cylmhd_set_initial(ctx, U, V))) return fail("set_initial", rc);
+            if (text_output(ctx, gdl, idl, outdir, n)) return 1;
+            printf("converted %s\n", dir);
+        }
+        cylmhd_destroy(ctx);
+        free(x); free(y); free(z); free(U); free(V);
+        return 0;
+    }
     if (restart >= 0) {
         /* main.c:250-270 */
         dirname_of(dir, sizeof dir, outdir, 0);
@@ -121,6 +176,7 @@ int main(int argc, char **argv)
         if ((rc = cylmhd_set_initial(ctx, U, V))) return fail("set_initial", rc);
         dirname_of(dir, sizeof dir, outdir, nout);
         if ((rc = cylmhd_binary_output(dir, NX, NY, NZ, t, x, y, z, V))) return fail(dir, rc);
+        if (text_output(ctx, gdl, idl, outdir, nout)) return 1;
     }
     printf("complete model setting.\n");
 
@@ -161,6 +217,7 @@ int main(int argc, char **argv)
             if ((rc = cylmhd_download_state(ctx, NULL, V))) return fail("download", rc);
             dirname_of(dir, sizeof dir, outdir, nout);
             if ((rc = cylmhd_binary_output(dir, NX, NY, NZ, t, x, y, z, V))) return fail(dir, rc);
+            if (text_output(ctx, gdl, idl, outdir, nout)) return 1;
         }
         if (t > tstop) {                                      /* main.c:383-390 */
             printf("time is over %.3e.\n", tstop);
