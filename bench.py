@@ -45,7 +45,38 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(cm, np, path, grid, steps, warmup, config=3):
+def time_text_output(cm, ctx):
+    """The text writers on the stepped state: one device-cut r-phi slice,
+    the IDL slice set and the GDLOutputCylinder set written to a temporary
+    directory, against bringing the whole state to the host."""
+    import shutil
+    import tempfile
+    res = {}
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.slice(cm.Q_RHO, cm.PLANE_RPHI, 1)
+    res["slice_rphi_ms"] = 1e3 * (time.perf_counter() - t0) / 3
+    td = tempfile.mkdtemp(prefix="cylmhd_text_")
+    try:
+        t0 = time.perf_counter()
+        ctx.idl_output(td, 1)
+        res["idl_set_ms"] = 1e3 * (time.perf_counter() - t0)
+        res["idl_set_bytes"] = sum(os.path.getsize(os.path.join(d, f)) for d, _, fs in os.walk(td) for f in fs)
+        t0 = time.perf_counter()
+        ctx.gdl_output(td, 1)
+        res["gdl_set_ms"] = 1e3 * (time.perf_counter() - t0)
+    finally:
+        shutil.rmtree(td, ignore_errors=True)
+    t0 = time.perf_counter()
+    U, V = ctx.download()
+    res["download_state_ms"] = 1e3 * (time.perf_counter() - t0)
+    del U, V
+    log(f"[bench] text output: slice {res['slice_rphi_ms']:.2f} ms, IDL set {res['idl_set_ms']:.0f} ms, "
+        f"GDL set {res['gdl_set_ms']:.0f} ms, full download {res['download_state_ms']:.0f} ms")
+    return res
+
+
+def run_path(cm, np, path, grid, steps, warmup, config=3, text=False):
     p = cm.params_config(config, *grid)
     p.path = path
     t0 = time.perf_counter()
@@ -74,6 +105,12 @@ def run_path(cm, np, path, grid, steps, warmup, config=3):
     kms, nlaunch = ctx.kernel_timing(0, reset=True)
     diag = ctx.diagnostics()
     counts = ctx.floor_counts()
+    text_res = None
+    if text:
+        try:
+            text_res = time_text_output(cm, ctx)
+        except Exception as e:                     # reported, never fatal to the GPU number
+            text_res = {"error": str(e)}
     ctx.close()
     ncell = p.ix * p.jy * p.kz
     name = cm.PATH_NAMES[path]
@@ -81,6 +118,8 @@ def run_path(cm, np, path, grid, steps, warmup, config=3):
                kernel_ms=kms, kernel_launches=nlaunch, interior_cells=ncell, t=t,
                mass=diag["mass"], energy=diag["energy"], divb_l1=diag["divb_l1"],
                pressure_floor_hits_last_step=int(counts[:, 1].sum()))
+    if text_res is not None:
+        res["text_output"] = text_res
     log(f"[bench] {name}: {res['value']:.4g} cell-updates/s, {res['ms_per_step']:.2f} ms/step, "
         f"stage kernel {kms:.3f} ms x {nlaunch}")
     return res
@@ -173,6 +212,7 @@ def main():
     ap.add_argument("--large-steps", type=int, default=10,
                     help="steps of the 768x768x384 disc with every source active, FP32 and FP64 "
                          "fused paths (0: skip)")
+    ap.add_argument("--no-text", action="store_true", help="skip timing the text writers")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run a plain device-to-device copy of this many MiB (the calibration "
                          "kernel of the FETCH_SIZE / WRITE_SIZE counter runs)")
@@ -195,7 +235,7 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] calibration copy: 3 x {a.copy_mib} MiB device-to-device")
         del src, dst
-    main_res = run_path(cm, np, cm.PATH_FUSED32, grid, a.steps, a.warmup)
+    main_res = run_path(cm, np, cm.PATH_FUSED32, grid, a.steps, a.warmup, text=not a.no_text)
     fp64 = {}
     if not a.no_fp64:
         k64 = max(2, a.steps // 2)
@@ -231,6 +271,8 @@ def main():
                    "grid": list(grid), "interior_cells": ncell, "path": "fused32"},
         "roofline": roofline,
     }
+    if "text_output" in main_res:
+        out["text_output"] = main_res["text_output"]
     if fp64:
         out["fp64"] = {k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms", "steps")} for k, v in fp64.items()}
         out["fp32_over_fp64"] = {k: main_res["value"] / v["value"] for k, v in fp64.items()}
