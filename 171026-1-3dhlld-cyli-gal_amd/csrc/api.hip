@@ -124,10 +124,6 @@ int validate(const cylmhd_params *p)
         snprintf(g_errmsg, sizeof g_errmsg, "unknown source %d", p->source);
         return CYLMHD_EINVAL;
     }
-    if (p->path != CYLMHD_PATH_PLAIN64 && p->limiter > CYLMHD_LIMITER_MC) {
-        snprintf(g_errmsg, sizeof g_errmsg, "the vanLeer and superbee limiters run on the plain FP64 path only");
-        return CYLMHD_EUNSUP;
-    }
     return CYLMHD_OK;
 }
 

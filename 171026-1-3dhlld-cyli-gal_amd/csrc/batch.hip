@@ -325,7 +325,7 @@ int build(cylmhd_batch *b, bool keep_time)
             a.NS = v.fs->NS; a.NZP = v.fs->NZP;
             a.rhofloor = (const float *)v.fs->rhofloor;
             a.Uinit = v.ds.Uinit;
-            a.Vinit_z = v.ds.Vinit_z;
+            a.Vinit_b = v.ds.Vinit_z;
             a.edge = (const float *)v.fs->edge;
             a.dtpart = nullptr;            /* set below */
             a.seglen = seg; a.tiles_j = tj; a.tiles_k = tk; a.ntiles = tj * tk;
@@ -477,9 +477,13 @@ int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out)
     *out = nullptr;
     for (int d = 0; d < n; d++) {
         if (p[d].path != CYLMHD_PATH_FUSED32 || p[d].limiter != p[0].limiter || p[d].solver != p[0].solver ||
-            p[d].rk != p[0].rk || p[d].limiter > CYLMHD_LIMITER_MC) {
-            set_errmsg("batch: every disc must use the FP32 fused path and the same limiter (minmod/MC), solver and RK");
+            p[d].rk != p[0].rk) {
+            set_errmsg("batch: every disc must use the FP32 fused path and the same limiter, solver and RK");
             return CYLMHD_EINVAL;
+        }
+        if (p[d].limiter > CYLMHD_LIMITER_MC) {
+            set_errmsg("batch: the batching form runs the minmod and MC limiters only");
+            return CYLMHD_EUNSUP;
         }
         if (p[d].source != CYLMHD_SOURCE_SPIRAL_COOLING) {
             set_errmsg("batch: the batching form runs the default source (SpiralGravityCooling) only");

@@ -30,7 +30,7 @@ namespace cyl {
 namespace FUSED_NS {
 
 /* ---- single-disc kernels: one work item per block --------------------- */
-template <typename T, bool MC, bool HLL, bool ALT>
+template <typename T, int LIM, bool HLL, bool ALT>
 __global__ __launch_bounds__((tile_threads<T, 0>()), (tile_waves<T, 0>())) void k_fused(FArgs<T> a)
 {
     const int b = remap_block(blockIdx.x, gridDim.x);
@@ -38,13 +38,13 @@ __global__ __launch_bounds__((tile_threads<T, 0>()), (tile_waves<T, 0>())) void 
         if (a.want_dt && threadIdx.x == 0) { a.dtpart[2 * blockIdx.x] = 0.0; a.dtpart[2 * blockIdx.x + 1] = 9.3e18; }
         return;
     }
-    fused_tile<T, MC, HLL, 0, ALT>(a, b % a.ntiles, b / a.ntiles, blockIdx.x);
+    fused_tile<T, LIM, HLL, 0, ALT>(a, b % a.ntiles, b / a.ntiles, blockIdx.x);
 }
 
-template <typename T, bool MC, bool HLL>
+template <typename T, int LIM, bool HLL>
 __global__ __launch_bounds__(256) void k_edge(FArgs<T> a, T *edge)
 {
-    edge_cell<T, MC, HLL>(a, edge, blockIdx.x * blockDim.x + threadIdx.x);
+    edge_cell<T, LIM, HLL>(a, edge, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 template <typename T>
@@ -298,7 +298,7 @@ static FArgs<T> make_args(FusedState &fs, const DevGeom &g, const DevState &d, c
     a.N = d.N;
     a.NS = fs.NS; a.NZP = fs.NZP;
     a.rhofloor = (const T *)fs.rhofloor;
-    a.Uinit = d.Uinit; a.Vinit_z = d.Vinit_z;
+    a.Uinit = d.Uinit; a.Vinit_b = d.Vinit_z;
     a.edge = (const T *)fs.edge;
     a.counts = d.counts;
     a.dtpart = fs.dtpart;
@@ -355,7 +355,7 @@ int setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, co
  * Un -> A) at each candidate segment count and keep the fastest.  Runs once
  * per context, inside the first step; the extra launches' floor counts are
  * cleared afterwards. */
-template <typename T, bool MC, bool HLL, bool ALT>
+template <typename T, int LIM, bool HLL, bool ALT>
 static void tune_segments(hipStream_t s, FusedState &fs, FArgs<T> a)
 {
     hipEvent_t e0, e1;
@@ -369,9 +369,9 @@ static void tune_segments(hipStream_t s, FusedState &fs, FArgs<T> a)
         a.seglen = (planes + ns - 1) / ns;
         a.nblocks = fs.tiles_j * fs.tiles_k * ((planes + a.seglen - 1) / a.seglen);
         const int nb = (a.nblocks + 7) / 8 * 8;
-        hipLaunchKernelGGL((k_fused<T, MC, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        hipLaunchKernelGGL((k_fused<T, LIM, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
         hipEventRecord(e0, s);
-        for (int r = 0; r < 2; r++) hipLaunchKernelGGL((k_fused<T, MC, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        for (int r = 0; r < 2; r++) hipLaunchKernelGGL((k_fused<T, LIM, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
         hipEventRecord(e1, s);
         float ms = 0.0f;
         if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
@@ -387,7 +387,7 @@ static void tune_segments(hipStream_t s, FusedState &fs, FArgs<T> a)
     hipMemsetAsync(a.counts, 0, 3 * sizeof(int), s);
 }
 
-template <typename T, bool MC, bool HLL, bool ALT>
+template <typename T, int LIM, bool HLL, bool ALT>
 static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
                     const StageArgs &sa, hipEvent_t ev0, hipEvent_t ev1)
 {
@@ -405,15 +405,16 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
     a.Uout = out;
     a.nblocks = fs.tiles_j * fs.tiles_k * fs.nseg;
     const int ne = (g.NY - 2) * (g.NZ - 2);
-    hipLaunchKernelGGL((k_edge<T, MC, HLL>), dim3((ne + 255) / 256), dim3(256), 0, s, a, (T *)fs.edge);
+    if (LIM == 2) a.Vinit_b = d.Vinit_y;          /* vanLeer: YVlrInitializeBoundary */
+    hipLaunchKernelGGL((k_edge<T, LIM, HLL>), dim3((ne + 255) / 256), dim3(256), 0, s, a, (T *)fs.edge);
     if (!fs.tuned && sa.stage == 1) {
-        tune_segments<T, MC, HLL, ALT>(s, fs, a);
+        tune_segments<T, LIM, HLL, ALT>(s, fs, a);
         a.seglen = fs.seglen;
         a.nblocks = fs.tiles_j * fs.tiles_k * fs.nseg;
     }
     const int nb = (a.nblocks + 7) / 8 * 8;
     if (ev0) hipEventRecord(ev0, s);
-    hipLaunchKernelGGL((k_fused<T, MC, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+    hipLaunchKernelGGL((k_fused<T, LIM, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
     if (ev1) hipEventRecord(ev1, s);
     const size_t nghost = 2 * (size_t)g.NY * g.NZ + (size_t)(g.NX - 2) * 2 * g.NZ + (size_t)(g.NX - 2) * (g.NY - 2) * 2;
     hipLaunchKernelGGL(k_halo<T>, dim3(nblocks(nghost, 256, 4096)), dim3(256), 0, s, a);
@@ -423,14 +424,23 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
 int stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
           const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1)
 {
-    /* the default source (SpiralGravityCooling) runs a kernel compiled for it alone */
-    if (c.source == 0) {
-        if (a.mc) { if (a.hll) stage_t<FT, true, true, false>(s, fs, g, d, c, a, ev0, ev1); else stage_t<FT, true, false, false>(s, fs, g, d, c, a, ev0, ev1); }
-        else      { if (a.hll) stage_t<FT, false, true, false>(s, fs, g, d, c, a, ev0, ev1); else stage_t<FT, false, false, false>(s, fs, g, d, c, a, ev0, ev1); }
-    } else {
-        if (a.mc) { if (a.hll) stage_t<FT, true, true, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<FT, true, false, true>(s, fs, g, d, c, a, ev0, ev1); }
-        else      { if (a.hll) stage_t<FT, false, true, true>(s, fs, g, d, c, a, ev0, ev1); else stage_t<FT, false, false, true>(s, fs, g, d, c, a, ev0, ev1); }
+    /* one kernel per limiter x solver; the default source (SpiralGravityCooling)
+     * runs a kernel compiled for it alone */
+#define CYL_STAGE(L, H, A) stage_t<FT, L, H, A>(s, fs, g, d, c, a, ev0, ev1)
+#define CYL_STAGE_LIM(H, A)                                  \
+    switch (a.limiter) {                                      \
+    case 0: CYL_STAGE(0, H, A); break;                        \
+    case 1: CYL_STAGE(1, H, A); break;                        \
+    case 2: CYL_STAGE(2, H, A); break;                        \
+    default: CYL_STAGE(3, H, A); break;                       \
     }
+    if (c.source == 0) {
+        if (a.hll) { CYL_STAGE_LIM(true, false) } else { CYL_STAGE_LIM(false, false) }
+    } else {
+        if (a.hll) { CYL_STAGE_LIM(true, true) } else { CYL_STAGE_LIM(false, true) }
+    }
+#undef CYL_STAGE_LIM
+#undef CYL_STAGE
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 

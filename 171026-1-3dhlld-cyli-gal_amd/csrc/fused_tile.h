@@ -31,11 +31,16 @@
  * in Check, the RK combination, the CFL reduction and the diagnostics in
  * FP64.
  *
- * Boundary faces: phi is periodic in both the cell and the interface
- * boundaries (YVlrPeriodic), so halo rows wrap (-1 -> NY-3, NY -> 2); the r
- * faces 0 and NX-2 both take (upper(NX-2), lower(1)) (XVlrPeriodic) and are
- * computed by edge_cell; the z faces 0 and NZ-2 follow
- * ZLeftVlVzAntisymBzsym / ZRightVrFree (minmod) or ZVlrFixed (MC).
+ * Boundary faces: phi is periodic in the cell boundaries, so halo rows wrap
+ * (-1 -> NY-3, NY -> 2).  The interface boundaries follow the limiter, as
+ * limiter{x,y,z}.h call them:
+ *   minmod   r XVlrPeriodic, phi YVlrPeriodic, z ZLeftVlVzAntisymBzsym /
+ *            ZRightVrFree
+ *   MC       r XVlrPeriodic, phi YVlrPeriodic, z ZVlrFixed (Vinitial)
+ *   vanLeer  r XVlrFree, phi YVlrInitialize (Vinitial at j = 0, NY-1),
+ *            z ZVlrFree
+ *   superbee r XVlrPeriodic, phi YVlrPeriodic, z ZVlrPeriodic
+ * The r faces 0 and NX-2 are computed by edge_cell before the stage kernel.
  */
 #ifndef CYLMHD_FUSED_TILE_H
 #define CYLMHD_FUSED_TILE_H
@@ -145,7 +150,10 @@ template <typename T> struct FArgs {
     const T *Uin, *Un;
     T *Uout, *shadow;
     const T *rhofloor;
-    const double *Uinit, *Vinit_z;
+    const double *Uinit;
+    /* initial-state planes an interface boundary reads: Vinit_z [9][2][NX][NY]
+     * (MC, ZVlrFixed) or Vinit_y [9][2][NX][NZ] (vanLeer, YVlrInitialize) */
+    const double *Vinit_b;
     size_t NS;                 /* component stride of the state arrays */
     int NZP;                   /* padded k-row length of the state arrays */
     const T *edge;
@@ -278,7 +286,7 @@ template <typename T, int SHAPE> constexpr int tile_waves() { return Tile<T, SHA
  * ibeg..iend of a TJ x TK (phi x z) column.  slot indexes the Setdt
  * partial the item writes on the last stage.  ALT: any source function
  * (Consts::source); false compiles SpiralGravityCooling only. */
-template <typename T, bool MC, bool HLL, int SHAPE, bool ALT>
+template <typename T, int LIM, bool HLL, int SHAPE, bool ALT>
 __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg, int slot)
 {
     constexpr int TJ = Tile<T, SHAPE>::TJ, TK = Tile<T, SHAPE>::TK;
@@ -286,7 +294,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
     constexpr int NZT = TJ * (TK + 2);           /* z-state tasks */
     constexpr int NCELL = TJ * TK, NHALO = 4 * TK + 4 * TJ;
     static_assert(NZT <= NTH && TJ * (TK + 1) <= NTH && NHALO <= NTH, "tile does not fit the block");
-    constexpr int LIM = MC ? 1 : 0;
+    constexpr bool MC = LIM == 1;
     /* padded plane: (TJ+4) x (TK+4) cells (corners unused), 9 components,
      * 3 ring slots; every stencil read is a per-thread base + an immediate */
     constexpr int PK = TK + 4, PS = (TJ + 4) * PK, SLOT = 9 * PS;
@@ -460,7 +468,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             const int side = t / (TJ * 9), q = t % (TJ * 9), rj = q / 9, m = q % 9;
             if (side < 2 && ((side == 0 && k0 == 1) || (side == 1 && kend == g.NZ - 2))) {
                 const size_t NP = (size_t)g.NX * g.NY;
-                s_vz[side][rj][m] = T(ro(a.Vinit_z, (size_t)m * 2 * NP + side * NP + (size_t)i * g.NY + jmap(g, j0 + rj)));
+                s_vz[side][rj][m] = T(ro(a.Vinit_b, (size_t)m * 2 * NP + side * NP + (size_t)i * g.NY + jmap(g, j0 + rj)));
             }
         }
         lds_barrier();
@@ -489,6 +497,19 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             const int qu = fyj * TK + fyk, ql = (fyj + 1) * TK + fyk;
 #pragma unroll
             for (int m = 0; m < 9; m++) { vl[m] = sy[0][m * NTH + qu]; vr[m] = sy[1][m * NTH + ql]; }
+            if (LIM == 2) {
+                /* YVlrInitializeBoundary: Vinitial at j = 0 is the left state
+                 * of face 0, at j = NY-1 the right state of face NY-2 */
+                const int jf = j0 + fyj - 1;
+                const size_t NQ = (size_t)g.NX * g.NZ, ik = (size_t)i * g.NZ + min(k0 + fyk, g.NZ - 1);
+                if (jf == 0) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vl[m] = T(ro(a.Vinit_b, (size_t)m * 2 * NQ + ik));
+                } else if (jf == g.NY - 2) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vr[m] = T(ro(a.Vinit_b, (size_t)m * 2 * NQ + NQ + ik));
+                }
+            }
             riemann<1, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fy);
 #pragma unroll
             for (int m = 0; m < 9; m++) sy[1][m * NTH + ql] = fy[m];
@@ -502,6 +523,16 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     /* ZVlrFixedBoundary: Vinitial at k = 0 */
 #pragma unroll
                     for (int m = 0; m < 9; m++) vl[m] = s_vz[0][fzj][m];
+                } else if (LIM == 2) {
+                    /* ZVlrFreeBoundary: Vl = Vr, the lower state of cell 1 */
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vl[m] = sz[1][m * NZT + ql];
+                } else if (LIM == 3) {
+                    /* ZVlrPeriodicBoundary: the upper state of cell NZ-2 */
+                    T vm[9], v0[9], vp[9];
+                    const int jj = jmap(g, j0 + fzj), L = g.NZ - 2;
+                    cell_V(a, SI(i, jj, L - 1), i, vm); cell_V(a, SI(i, jj, L), i, v0); cell_V(a, SI(i, jj, L + 1), i, vp);
+                    cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), vl, (T *)nullptr);
                 } else {
                     /* ZLeftVlVzAntisymBzsym: lower state of cell 2, vz negated */
                     T vm[9], v0[9], vp[9];
@@ -519,8 +550,14 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     /* ZVlrFixedBoundary: Vinitial at k = NZ-1 */
 #pragma unroll
                     for (int m = 0; m < 9; m++) vr[m] = s_vz[1][fzj][m];
+                } else if (LIM == 3) {
+                    /* ZVlrPeriodicBoundary: the lower state of cell 1 */
+                    T vm[9], v0[9], vp[9];
+                    const int jj = jmap(g, j0 + fzj);
+                    cell_V(a, SI(i, jj, 0), i, vm); cell_V(a, SI(i, jj, 1), i, v0); cell_V(a, SI(i, jj, 2), i, vp);
+                    cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), (T *)nullptr, vr);
                 } else {
-                    /* ZRightVrFreeBoundary: Vr = Vl */
+                    /* ZRightVrFreeBoundary (minmod) / ZVlrFreeBoundary (vanLeer): Vr = Vl */
 #pragma unroll
                     for (int m = 0; m < 9; m++) vr[m] = vl[m];
                 }
@@ -698,8 +735,11 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
     }
 }
 
-/* ---- r-boundary faces 0 and NX-2: (upper(NX-2), lower(1)) ---------------- */
-template <typename T, bool MC, bool HLL>
+/* ---- r-boundary faces 0 and NX-2 ------------------------------------------
+ * XVlrPeriodicBoundary (minmod, MC, superbee): both faces take
+ * (upper(NX-2), lower(1)); XVlrFreeBoundary (vanLeer): face 0 takes
+ * (lower(1), lower(1)), face NX-2 (upper(NX-2), upper(NX-2)). */
+template <typename T, int LIM, bool HLL>
 __device__ __forceinline__ void edge_cell(const FArgs<T> &a, T *edge, int q)
 {
     const DevGeom &g = a.g;
@@ -709,14 +749,14 @@ __device__ __forceinline__ void edge_cell(const FArgs<T> &a, T *edge, int q)
     T vm[9], v0[9], vp[9], up[9], lo[9], f[9];
     const int L = g.NX - 2;
     cell_V(a, SI(L - 1, j, k), L - 1, vm); cell_V(a, SI(L, j, k), L, v0); cell_V(a, SI(L + 1, j, k), L + 1, vp);
-    cell_states<0, MC>(a.c, vm, v0, vp, T(g.oneodx[L]), T(g.oneodx[L - 1]), T(1.0), up, (T *)nullptr);
+    cell_states<0, LIM>(a.c, vm, v0, vp, T(g.oneodx[L]), T(g.oneodx[L - 1]), T(1.0), up, (T *)nullptr);
     cell_V(a, SI(0, j, k), 0, vm); cell_V(a, SI(1, j, k), 1, v0); cell_V(a, SI(2, j, k), 2, vp);
-    cell_states<0, MC>(a.c, vm, v0, vp, T(g.oneodx[1]), T(g.oneodx[0]), T(1.0), (T *)nullptr, lo);
+    cell_states<0, LIM>(a.c, vm, v0, vp, T(g.oneodx[1]), T(g.oneodx[0]), T(1.0), (T *)nullptr, lo);
     const size_t NE = (size_t)g.NY * g.NZ, e = (size_t)j * g.NZ + k;
-    riemann<0, HLL>(a.c, up, lo, T(g.xm[0]), T(g.oneox[0]), f);
+    riemann<0, HLL>(a.c, LIM == 2 ? lo : up, lo, T(g.xm[0]), T(g.oneox[0]), f);
 #pragma unroll
     for (int m = 0; m < 9; m++) edge[m * NE + e] = f[m];
-    riemann<0, HLL>(a.c, up, lo, T(g.xm[L]), T(g.oneox[L]), f);
+    riemann<0, HLL>(a.c, up, LIM == 2 ? up : lo, T(g.xm[L]), T(g.oneox[L]), f);
 #pragma unroll
     for (int m = 0; m < 9; m++) edge[9 * NE + m * NE + e] = f[m];
 }

@@ -38,8 +38,8 @@ extern "C" {
 /* limiter: firststep.h:37-38 (MinmodLimiterX is the active one)          */
 #define CYLMHD_LIMITER_MINMOD 0   /* limiterx.h:171-217 (MUSCL-Hancock form) */
 #define CYLMHD_LIMITER_MC     1   /* limiterx.h:335-385                      */
-#define CYLMHD_LIMITER_VANLEER  2 /* limiterx.h:266-326 (plain FP64 path)    */
-#define CYLMHD_LIMITER_SUPERBEE 3 /* limiterx.h:394-449 (plain FP64 path)    */
+#define CYLMHD_LIMITER_VANLEER  2 /* limiterx.h:266-326                      */
+#define CYLMHD_LIMITER_SUPERBEE 3 /* limiterx.h:394-449                      */
 /* Riemann solver: firststep.h:45-46                                       */
 #define CYLMHD_SOLVER_HLLD    0   /* hlldx.h:28-499                          */
 #define CYLMHD_SOLVER_HLL     1   /* hllx.h:29-184                           */
@@ -191,8 +191,9 @@ typedef struct cylmhd_batch cylmhd_batch;
  * annulus, spiral amplitude uniform in [0, 0.1], its own seed; minmod +
  * HLLD, RK3, FP32 path.  p must hold n entries. */
 int cylmhd_params_batch(cylmhd_params *p, int n, uint64_t seed);
-/* All discs must share limiter (minmod or MC), solver and RK order, and use
- * the default source (CYLMHD_SOURCE_SPIRAL_COOLING; others: CYLMHD_EUNSUP). */
+/* All discs must share limiter, solver and RK order (else CYLMHD_EINVAL);
+ * the limiter must be minmod or MC and the source the default
+ * (CYLMHD_SOURCE_SPIRAL_COOLING): others give CYLMHD_EUNSUP. */
 int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out);
 int cylmhd_batch_destroy(cylmhd_batch *b);
 /* The context of disc d (owned by the batch; do not destroy it). */

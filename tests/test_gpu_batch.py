@@ -109,3 +109,14 @@ def test_batch_stops_only_the_disc_below_dtmin(gpu):
                 assert not st["alive"] and st["steps"] == 0
             else:
                 assert st["alive"] and st["steps"] == 2
+
+
+@pytest.mark.parametrize("limiter", [cm.LIMITER_VANLEER, cm.LIMITER_SUPERBEE], ids=["vanleer", "superbee"])
+def test_batch_rejects_vanleer_superbee(gpu, limiter):
+    """The batching form runs minmod and MC only (single discs run every limiter)."""
+    ps = batch_params()
+    for p in ps:
+        p.limiter = limiter
+    with pytest.raises(cm.CylmhdError) as e:
+        cm.Batch(ps)
+    assert e.value.code == cm.EUNSUP

@@ -166,12 +166,43 @@ def test_totals_100_steps(gpu, path):
     g.close()
 
 
-@pytest.mark.parametrize("path", [cm.PATH_FUSED64, cm.PATH_FUSED32])
-def test_fused_rejects_vanleer_superbee(gpu, path):
-    for lim in (cm.LIMITER_VANLEER, cm.LIMITER_SUPERBEE):
-        p = cm.params_config(2, 16, 16, 10)
-        p.limiter = lim
-        p.path = path
-        with pytest.raises(cm.CylmhdError) as e:
-            cm.Context(p)
-        assert e.value.code == cm.EUNSUP
+VL_SB = [cm.LIMITER_VANLEER, cm.LIMITER_SUPERBEE]
+
+
+@pytest.mark.parametrize("limiter", VL_SB, ids=["vanleer", "superbee"])
+@pytest.mark.parametrize("solver", [cm.SOLVER_HLLD, cm.SOLVER_HLL], ids=["hlld", "hll"])
+def test_fused64_vanleer_superbee_disc(gpu, limiter, solver):
+    """vanLeer (XVlrFree, YVlrInitialize, ZVlrFree) and superbee (ZVlrPeriodic)
+    interface boundaries on the fused FP64 path, z tiles split (nz = 34)."""
+    p, Ui, Vi, Uc, Vc = disc_case(2, 32, 32, 34, limiter=limiter, solver=solver)
+    x = cm.grid(p)[0]
+    hist = run_pair(p, Ui, Vi, Uc, Vc, 2, cm.PATH_FUSED64)
+    for n, h in enumerate(hist):
+        e = rel_err_scaled(h["Ug"], h["Uo"], x)
+        print(f"step {n}: scaled eU={e.max():.2e}")
+        assert e.max() <= TOL_STEP, e
+        assert abs(h["dtg"] - h["dto"]) <= 1e-12 * h["dto"]
+
+
+@pytest.mark.parametrize("rk", [3, 2])
+@pytest.mark.parametrize("limiter", VL_SB, ids=["vanleer", "superbee"])
+def test_fused64_vanleer_superbee_random_state(gpu, limiter, rk):
+    """A random state puts different values on every boundary plane, so a
+    wrong interface boundary (periodic instead of free, the wrong Vinitial
+    plane, a shifted wrap) shows at the 1e-12 level."""
+    p, Ui, Vi, Uc, Vc = rand_case(rk=rk, limiter=limiter, consistent=True, nr=40, nphi=32, nz=34)
+    hist = run_pair(p, Ui, Vi, Uc, Vc, 1, cm.PATH_FUSED64)
+    h = hist[0]
+    print(f"eU={h['eU']}")
+    assert h["eU"].max() <= TOL_STEP, h["eU"]
+
+
+@pytest.mark.parametrize("limiter", VL_SB, ids=["vanleer", "superbee"])
+def test_fused32_vanleer_superbee_disc(gpu, limiter):
+    p, Ui, Vi, Uc, Vc = disc_case(3, 32, 32, 34, limiter=limiter)
+    x = cm.grid(p)[0]
+    hist = run_pair(p, Ui, Vi, Uc, Vc, 3, cm.PATH_FUSED32)
+    for n, h in enumerate(hist):
+        e = l1_rel_scaled(h["Ug"], h["Uo"], x)
+        print(f"step {n}: L1 rel {e.max():.2e}")
+        assert e.max() <= 1.0e-5, e
