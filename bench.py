@@ -209,7 +209,7 @@ def main():
                     help="steps of the FP32-vs-FP64 error report (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=200,
                     help="steps of the 48-disc batching form (0: skip)")
-    ap.add_argument("--large-steps", type=int, default=10,
+    ap.add_argument("--large-steps", type=int, default=50,
                     help="steps of the 768x768x384 disc with every source active, FP32 and FP64 "
                          "fused paths (0: skip)")
     ap.add_argument("--no-text", action="store_true", help="skip timing the text writers")

@@ -145,6 +145,21 @@ def test_fused32_source_variants(gpu, source):
         assert e.max() <= 1.0e-5, e
 
 
+@pytest.mark.parametrize("path", [cm.PATH_PLAIN64, cm.PATH_FUSED64], ids=["plain64", "fused64"])
+def test_full_size_256_one_step(gpu, path):
+    """The 256x256x128 disc (MC + HLLD, spiral m = 2 at 5 %) at full size:
+    one RK3 step against the oracle (16 threads on the host), 1e-12 per
+    conserved variable, dt and its cell equal."""
+    p, Ui, Vi, Uc, Vc = disc_case(2, 256, 256, 128)
+    x = cm.grid(p)[0]
+    hist = run_pair(p, Ui, Vi, Uc, Vc, 1, path)
+    h = hist[0]
+    e = rel_err_scaled(h["Ug"], h["Uo"], x)
+    print(f"{cm.PATH_NAMES[path]}: scaled eU={e.max():.2e} dt {h['dtg']:.17g} {h['dto']:.17g}")
+    assert e.max() <= TOL_STEP, e
+    assert abs(h["dtg"] - h["dto"]) <= 1e-14 * h["dto"] and h["ijkg"] == h["ijko"]
+
+
 @pytest.mark.parametrize("path", [cm.PATH_PLAIN64, cm.PATH_FUSED64])
 def test_totals_100_steps(gpu, path):
     """Total mass, energy and L1 div B tracked to 1e-10 relative over 100
