@@ -15,7 +15,10 @@ struct FusedState {
     size_t esize = 8;          /* bytes per state value */
     /* state buffers, [9][N] of float or double:
      *   Un    U^n (in/out of a step)      A, B  stage outputs U(1), U(2)
-     *   shadow  pre-Check stage values of the cells the ghost map reads */
+     *   shadow  pre-Check stage values of the cells the ghost map reads:
+     *           [9][NS] for the r and phi sides, then [3][NX][NY][9] for the
+     *           z sides (planes k = 1, 2, NZ-2, packed so that the ghost pass
+     *           reads them from consecutive addresses) */
     void *Un = nullptr, *A = nullptr, *B = nullptr, *shadow = nullptr;
     void *rhofloor = nullptr;  /* [N] rhocr * Uinitial[0] */
     void *edge = nullptr;      /* [2][9][NY][NZ] x-boundary face fluxes */

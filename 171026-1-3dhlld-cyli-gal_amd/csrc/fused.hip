@@ -227,7 +227,7 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     const size_t NS = fs.NS;
     int rc;
     if ((rc = alloc(9 * NS * fs.esize, &fs.Un)) || (rc = alloc(9 * NS * fs.esize, &fs.A)) ||
-        (rc = alloc(9 * NS * fs.esize, &fs.B)) || (rc = alloc(9 * NS * fs.esize, &fs.shadow)) ||
+        (rc = alloc(9 * NS * fs.esize, &fs.B)) || (rc = alloc((9 * NS + 27 * (size_t)g.NX * g.NY) * fs.esize, &fs.shadow)) ||
         (rc = alloc(NS * fs.esize, &fs.rhofloor)) || (rc = alloc(2 * 9 * (size_t)g.NY * g.NZ * fs.esize, &fs.edge)))
         return rc;
     void *p;

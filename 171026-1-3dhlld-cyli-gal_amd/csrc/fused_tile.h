@@ -660,9 +660,27 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                 u[m] = T(kf * (kb * unm + ke * sv));
             }
             if (!a.rk3 && gk == 1) u[3] = T(0.0);       /* ZLeftVzAntisymBSym, U3[k=1] */
-            if (i == 1 || i == g.NX - 2 || gj == 1 || gj == g.NY - 2 || gk <= 2 || gk == g.NZ - 2) {
+            if (i == 1 || i == g.NX - 2 || gj == 1 || gj == g.NY - 2) {
 #pragma unroll
                 for (int m = 0; m < 9; m++) a.shadow[m * NS + cc] = u[m];
+            }
+            if (gk <= 2 || gk == g.NZ - 2) {
+                /* z sides: packed planes k = 1, 2, NZ-2 after the shadow,
+                 * the nine values of a cell together */
+                const size_t P = (size_t)g.NX * g.NY;
+                T *zs = a.shadow + 9 * NS + 9 * ((size_t)i * g.NY + gj);
+                if (gk == 1) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) zs[m] = u[m];
+                }
+                if (gk == 2) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) zs[9 * P + m] = u[m];
+                }
+                if (gk == g.NZ - 2) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) zs[18 * P + m] = u[m];
+                }
             }
             if (a.last) {
                 const bool ax = i >= g.NX - 11, ac = i <= g.irin, az = !a.rk3 && gk >= g.NZ - 11;
@@ -789,11 +807,18 @@ __device__ __forceinline__ void halo_range(const FArgs<T> &a, size_t first, size
         unsigned sign;
         bool zero3;
         ghost_source(g, a.rk3, i, j, k, si, sj, sk, sign, zero3);
-        const size_t src = SI(si, sj, sk), dst = SI(i, j, k), dstr = C3(i, j, k);
+        const size_t dst = SI(i, j, k), dstr = C3(i, j, k);
+        /* the z-side ghosts of the interior columns read the packed planes
+         * k = 1, 2, NZ-2; the others the shadow in the state layout */
+        const bool zside = (k == 0 || k == (int)NZ - 1) && i >= 1 && i <= (int)NX - 2 && j >= 1 && j <= (int)NY - 2;
+        const size_t P = NX * NY;
+        const int zsl = sk == 1 ? 0 : (sk == 2 ? 1 : 2);
+        const T *sp = zside ? a.shadow + 9 * NS + 9 * ((size_t)zsl * P + (size_t)si * NY + sj) : a.shadow + SI(si, sj, sk);
+        const size_t sstride = zside ? 1 : NS;
         T u[9], v[9];
 #pragma unroll
         for (int m = 0; m < 9; m++) {
-            const T val = a.shadow[m * NS + src];
+            const T val = sp[m * sstride];
             u[m] = (sign >> m) & 1u ? -val : val;
         }
         if (zero3) u[3] = T(0.0);
@@ -812,8 +837,22 @@ __device__ __forceinline__ void halo_range(const FArgs<T> &a, size_t first, size
         Prec<T>::utov(a.c, u, g.oneox[i], v);
         const int fl = Prec<T>::check(a.c, u, v, g.x[i], a.rhofloor[dst]);
         f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
+        /* A z-side ghost is the only cell of its 64-byte line (the rest is
+         * row padding: KOFF cells before k = 1, NZP - NZ - KOFF after NZ-1),
+         * so the whole line is written, avoiding a partial-line write. */
+        constexpr int E = 64 / (int)sizeof(T);
+        if ((k == 0 && (dst % E) == E - 1) || (k == (int)NZ - 1 && (dst % E) == 0)) {
+            const size_t line = dst - dst % E;
 #pragma unroll
-        for (int m = 0; m < 9; m++) a.Uout[m * NS + dst] = u[m];
+            for (int m = 0; m < 9; m++) {
+                T *q = a.Uout + m * NS + line;
+#pragma unroll
+                for (int e = 0; e < E; e++) q[e] = u[m];
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 9; m++) a.Uout[m * NS + dst] = u[m];
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         f0 += __shfl_xor(f0, off); f1 += __shfl_xor(f1, off); f2 += __shfl_xor(f2, off);
