@@ -111,19 +111,13 @@ __global__ __launch_bounds__(256) void k_setdt_batch(const FArgs<float> *__restr
         const double tt = rmax(t1, tc);
         if ((tt > best || (tt == best && (long long)cc < bix)) && tt > 0.0) { best = tt; bix = (long long)cc; }
     }
-    __shared__ double sv[256];
-    __shared__ long long sx[256];
-    const int t = threadIdx.x;
-    sv[t] = best; sx[t] = bix;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (t < o && (sv[t + o] > sv[t] || (sv[t + o] == sv[t] && sx[t + o] < sx[t]))) { sv[t] = sv[t + o]; sx[t] = sx[t + o]; }
-        __syncthreads();
-    }
-    if (t == 0) {
+    __shared__ double sv[4];
+    __shared__ long long sx[4];
+    block_dt_max<4>(best, bix, sv, sx);
+    if (threadIdx.x == 0) {
         const int slot = bd[d].item_lo + (int)(blockIdx.x - off[d]);
-        dtpart[2 * slot] = sv[0];
-        dtpart[2 * slot + 1] = (double)sx[0];
+        dtpart[2 * slot] = best;
+        dtpart[2 * slot + 1] = (double)bix;
     }
 }
 
@@ -144,16 +138,11 @@ __global__ __launch_bounds__(256) void k_dt_batch(FArgs<float> *__restrict__ des
         const long long ix = dtpart[2 * q + 1] >= 9.0e18 ? LLONG_MAX : (long long)dtpart[2 * q + 1];
         if ((v > best || (v == best && ix < bix)) && v > 0.0) { best = v; bix = ix; }
     }
-    __shared__ double sv[256];
-    __shared__ long long sx[256];
-    sv[t] = best; sx[t] = bix;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (t < o && (sv[t + o] > sv[t] || (sv[t + o] == sv[t] && sx[t + o] < sx[t]))) { sv[t] = sv[t + o]; sx[t] = sx[t + o]; }
-        __syncthreads();
-    }
+    __shared__ double sv[4];
+    __shared__ long long sx[4];
+    block_dt_max<4>(best, bix, sv, sx);
     if (t == 0) {
-        const double dtmax = sv[0];
+        const double dtmax = best;
         const double dt = D.cfl / dtmax, ch = D.minminlength * dtmax;
         if (!(dt >= D.dtmin) || isinf(dt)) {
             D.alive = 0;

@@ -11,6 +11,8 @@
 #ifndef CYLMHD_CTX_H
 #define CYLMHD_CTX_H
 
+#include <climits>
+
 #include <hip/hip_runtime.h>
 
 #include "../../include/cylmhd.h"
@@ -95,6 +97,43 @@ __device__ __forceinline__ void ghost_source(const DevGeom &g, bool rk3, int i, 
     } else {
         si = i == NX - 1 ? NX - 2 : i;
         sj = j == 0 ? NY - 2 : (j == NY - 1 ? 1 : j);
+    }
+}
+
+/* ---- Setdt's maximum (setdt.h:52-69) ----------------------------------
+ * The largest 1/dt and the first cell (lowest reference-layout index) that
+ * attains it.  The pair order (value descending, index ascending) is a total
+ * order, so the result does not depend on the reduction order. */
+__device__ __forceinline__ bool dt_better(double v, long long ix, double bv, long long bix)
+{
+    return v > bv || (v == bv && ix < bix);
+}
+
+/* within a wavefront: xor butterfly over the 64 lanes (every lane must be
+ * active), every lane ends with the wavefront's maximum */
+__device__ __forceinline__ void wave_dt_max(double &v, long long &ix)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(v, off);
+        const long long oi = __shfl_xor(ix, off);
+        if (dt_better(ov, oi, v, ix)) { v = ov; ix = oi; }
+    }
+}
+
+/* across a block of NW wavefronts (all threads call it): wavefront maxima
+ * through NW LDS slots, then wavefront 0; thread 0 returns the block's */
+template <int NW>
+__device__ __forceinline__ void block_dt_max(double &v, long long &ix, double *sv, long long *sx)
+{
+    wave_dt_max(v, ix);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { sv[w] = v; sx[w] = ix; }
+    __syncthreads();
+    if (w == 0) {
+        v = lane < NW ? sv[lane] : 0.0;
+        ix = lane < NW ? sx[lane] : LLONG_MAX;
+        wave_dt_max(v, ix);
     }
 }
 

@@ -80,37 +80,26 @@ __global__ __launch_bounds__(256) void k_setdt_state(FArgs<T> a)
         const double tt = rmax(t1, tc);
         if ((tt > best || (tt == best && (long long)cc < bix)) && tt > 0.0) { best = tt; bix = (long long)cc; }
     }
-    __shared__ double sv[256];
-    __shared__ long long sx[256];
-    const int t = threadIdx.x;
-    sv[t] = best; sx[t] = bix;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (t < off && (sv[t + off] > sv[t] || (sv[t + off] == sv[t] && sx[t + off] < sx[t]))) { sv[t] = sv[t + off]; sx[t] = sx[t + off]; }
-        __syncthreads();
-    }
-    if (t == 0) { a.dtpart[2 * blockIdx.x] = sv[0]; a.dtpart[2 * blockIdx.x + 1] = (double)sx[0]; }
+    __shared__ double sv[4];
+    __shared__ long long sx[4];
+    block_dt_max<4>(best, bix, sv, sx);
+    if (threadIdx.x == 0) { a.dtpart[2 * blockIdx.x] = best; a.dtpart[2 * blockIdx.x + 1] = (double)bix; }
 }
 
 __global__ __launch_bounds__(256) void k_dt_final(const double *__restrict__ part, int nb, double *__restrict__ out)
 {
-    __shared__ double sv[256];
-    __shared__ long long sx[256];
+    __shared__ double sv[4];
+    __shared__ long long sx[4];
     const int t = threadIdx.x;
     double best = 0.0;
     long long bix = LLONG_MAX;
     for (int b = t; b < nb; b += 256) {
         const double v = part[2 * b];
         const long long ix = part[2 * b + 1] >= 9.0e18 ? LLONG_MAX : (long long)part[2 * b + 1];
-        if ((v > best || (v == best && ix < bix)) && v > 0.0) { best = v; bix = ix; }
+        if (dt_better(v, ix, best, bix) && v > 0.0) { best = v; bix = ix; }
     }
-    sv[t] = best; sx[t] = bix;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (t < off && (sv[t + off] > sv[t] || (sv[t + off] == sv[t] && sx[t + off] < sx[t]))) { sv[t] = sv[t + off]; sx[t] = sx[t + off]; }
-        __syncthreads();
-    }
-    if (t == 0) { out[0] = sv[0]; out[1] = sx[0] == LLONG_MAX ? -1.0 : (double)sx[0]; }
+    block_dt_max<4>(best, bix, sv, sx);
+    if (t == 0) { out[0] = best; out[1] = bix == LLONG_MAX ? -1.0 : (double)bix; }
 }
 
 /* ---- conversions and diagnostics ---------------------------------------- */

@@ -736,20 +736,14 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         if (f2) atomicAdd(&a.counts[2], f2);
     }
     if (a.want_dt) {
+        /* Setdt partial of the work item: lane shuffles, then one LDS slot
+         * per wavefront (sy is free: the last update has read it) */
+        static_assert(NTH % 64 == 0, "whole wavefronts");
         __syncthreads();
         double *sv = (double *)&sy[0][0];
-        long long *sx = (long long *)(sv + NTH);
-        sv[t] = best; sx[t] = bix;
-        __syncthreads();
-        for (int off = NTH / 2; off > 0; off >>= 1) {
-            if (t < off) {
-                const double v = sv[t + off];
-                const long long ix = sx[t + off];
-                if (v > sv[t] || (v == sv[t] && ix < sx[t])) { sv[t] = v; sx[t] = ix; }
-            }
-            __syncthreads();
-        }
-        if (t == 0) { a.dtpart[2 * slot] = sv[0]; a.dtpart[2 * slot + 1] = (double)sx[0]; }
+        long long *sx = (long long *)(sv + NTH / 64);
+        block_dt_max<NTH / 64>(best, bix, sv, sx);
+        if (t == 0) { a.dtpart[2 * slot] = best; a.dtpart[2 * slot + 1] = (double)bix; }
     }
 }
 

@@ -364,21 +364,15 @@ void launch_sincos(hipStream_t s, const DevGeom &g, double *sc, const double *lo
  * it; FP64, deterministic (block partials, then one block) --------------- */
 __device__ __forceinline__ bool better(double v, long long ix, double bv, long long bix)
 {
-    return v > bv || (v == bv && ix < bix);
+    return dt_better(v, ix, bv, bix);
 }
 
+/* 256-thread blocks: lane shuffles within each wavefront, then 4 LDS slots */
 __device__ void block_argmax(double &v, long long &ix)
 {
-    __shared__ double sv[256];
-    __shared__ long long si[256];
-    const int t = threadIdx.x;
-    sv[t] = v; si[t] = ix;
-    __syncthreads();
-    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-        if (t < off && better(sv[t + off], si[t + off], sv[t], si[t])) { sv[t] = sv[t + off]; si[t] = si[t + off]; }
-        __syncthreads();
-    }
-    v = sv[0]; ix = si[0];
+    __shared__ double sv[4];
+    __shared__ long long si[4];
+    block_dt_max<4>(v, ix, sv, si);
 }
 
 __global__ __launch_bounds__(256) void k_setdt(DevGeom g, Consts c, const double *__restrict__ V, size_t N,
