@@ -53,7 +53,7 @@ __device__ __forceinline__ int find_disc(const int *__restrict__ off, int n, int
     return lo;
 }
 
-template <bool MC, bool HLL, int S>
+template <int LIM, bool HLL, int S, bool ALT>
 __global__ __launch_bounds__((tile_threads<float, S>()), (tile_waves<float, S>())) void k_fused_batch(
     const FArgs<float> *__restrict__ descs, const int4 *__restrict__ items, int nitems, const BDisc *__restrict__ bd)
 {
@@ -61,17 +61,17 @@ __global__ __launch_bounds__((tile_threads<float, S>()), (tile_waves<float, S>()
     for (int it = b; it < nitems; it += gridDim.x) {
         const int4 w = items[it];
         if (!bd[w.x].alive) continue;
-        fused_tile<float, MC, HLL, S, false>(descs[w.x], w.y, w.z, it);   /* default source only */
+        fused_tile<float, LIM, HLL, S, ALT>(descs[w.x], w.y, w.z, it);
     }
 }
 
-template <bool MC, bool HLL>
+template <int LIM, bool HLL>
 __global__ __launch_bounds__(256) void k_edge_batch(const FArgs<float> *__restrict__ descs, const int *__restrict__ off,
                                                     int n, const BDisc *__restrict__ bd)
 {
     const int d = find_disc(off, n, blockIdx.x);
     if (!bd[d].alive) return;
-    edge_cell<float, MC, HLL>(descs[d], bd[d].edge, (blockIdx.x - off[d]) * 256 + threadIdx.x);
+    edge_cell<float, LIM, HLL>(descs[d], bd[d].edge, (blockIdx.x - off[d]) * 256 + threadIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_halo_batch(const FArgs<float> *__restrict__ descs, const int *__restrict__ off,
@@ -219,7 +219,9 @@ struct cylmhd_batch {
     std::vector<cylmhd_ctx *> disc;
     std::vector<DiscView> view;
     hipStream_t stream = nullptr;
-    bool mc = false, hll = false, rk3 = true;
+    int lim = 0;               /* CYLMHD_LIMITER_*, shared by every disc */
+    bool alt = false;          /* some disc runs a source other than the default */
+    bool hll = false, rk3 = true;
     int nst = 3;
     /* tuned launch shape */
     int shape = 0, seglen = 0, grid = 1024;
@@ -314,7 +316,7 @@ int build(cylmhd_batch *b, bool keep_time)
             a.NS = v.fs->NS; a.NZP = v.fs->NZP;
             a.rhofloor = (const float *)v.fs->rhofloor;
             a.Uinit = v.ds.Uinit;
-            a.Vinit_b = v.ds.Vinit_z;
+            a.Vinit_b = b->lim == CYLMHD_LIMITER_VANLEER ? v.ds.Vinit_y : v.ds.Vinit_z;
             a.edge = (const float *)v.fs->edge;
             a.dtpart = nullptr;            /* set below */
             a.seglen = seg; a.tiles_j = tj; a.tiles_k = tk; a.ntiles = tj * tk;
@@ -390,17 +392,17 @@ int build(cylmhd_batch *b, bool keep_time)
     return CYLMHD_OK;
 }
 
-template <bool MC, bool HLL>
+template <int LIM, bool HLL, bool ALT>
 void launch_stage(cylmhd_batch *b, int s)
 {
     const FArgs<float> *descs = b->d_descs + (size_t)s * b->n;
-    hipLaunchKernelGGL((k_edge_batch<MC, HLL>), dim3(b->eblocks), dim3(256), 0, b->stream, descs, b->d_eoff, b->n, b->d_bd);
+    hipLaunchKernelGGL((k_edge_batch<LIM, HLL>), dim3(b->eblocks), dim3(256), 0, b->stream, descs, b->d_eoff, b->n, b->d_bd);
     const int G = std::max(8, std::min((b->nitems + 7) / 8 * 8, b->grid));
     if (b->shape)
-        hipLaunchKernelGGL((k_fused_batch<MC, HLL, 1>), dim3(G), dim3(tile_threads<float, 1>()), 0, b->stream, descs,
+        hipLaunchKernelGGL((k_fused_batch<LIM, HLL, 1, ALT>), dim3(G), dim3(tile_threads<float, 1>()), 0, b->stream, descs,
                            b->d_items, b->nitems, b->d_bd);
     else
-        hipLaunchKernelGGL((k_fused_batch<MC, HLL, 0>), dim3(G), dim3(tile_threads<float, 0>()), 0, b->stream, descs,
+        hipLaunchKernelGGL((k_fused_batch<LIM, HLL, 0, ALT>), dim3(G), dim3(tile_threads<float, 0>()), 0, b->stream, descs,
                            b->d_items, b->nitems, b->d_bd);
     hipLaunchKernelGGL(k_halo_batch, dim3(b->hblocks), dim3(256), 0, b->stream, descs, b->d_hoff, b->n, b->d_bd);
 }
@@ -410,10 +412,25 @@ void enqueue_step(cylmhd_batch *b)
 {
     hipLaunchKernelGGL(k_dt_batch, dim3(b->n), dim3(256), 0, b->stream, b->d_descs, b->nst, b->n, b->d_bd, b->d_dtpart);
     hipLaunchKernelGGL(k_sincos_batch, dim3(b->sblocks), dim3(256), 0, b->stream, b->d_descs, b->d_soff, b->n, b->d_bd);
-    for (int s = 0; s < b->nst; s++) {
-        if (b->mc) { if (b->hll) launch_stage<true, true>(b, s); else launch_stage<true, false>(b, s); }
-        else       { if (b->hll) launch_stage<false, true>(b, s); else launch_stage<false, false>(b, s); }
+    /* one kernel per limiter x solver; discs that all run the default
+     * source (SpiralGravityCooling) use a kernel compiled for it alone */
+#define CYL_BSTAGE(L, H, A) launch_stage<L, H, A>(b, s)
+#define CYL_BSTAGE_LIM(H, A)                                   \
+    switch (b->lim) {                                           \
+    case 0: CYL_BSTAGE(0, H, A); break;                         \
+    case 1: CYL_BSTAGE(1, H, A); break;                         \
+    case 2: CYL_BSTAGE(2, H, A); break;                         \
+    default: CYL_BSTAGE(3, H, A); break;                        \
     }
+    for (int s = 0; s < b->nst; s++) {
+        if (b->alt) {
+            if (b->hll) { CYL_BSTAGE_LIM(true, true) } else { CYL_BSTAGE_LIM(false, true) }
+        } else {
+            if (b->hll) { CYL_BSTAGE_LIM(true, false) } else { CYL_BSTAGE_LIM(false, false) }
+        }
+    }
+#undef CYL_BSTAGE_LIM
+#undef CYL_BSTAGE
     hipLaunchKernelGGL(k_time_batch, dim3(cdiv(b->n, 64)), dim3(64), 0, b->stream, b->d_bd, b->n);
 }
 
@@ -470,18 +487,11 @@ int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out)
             set_errmsg("batch: every disc must use the FP32 fused path and the same limiter, solver and RK");
             return CYLMHD_EINVAL;
         }
-        if (p[d].limiter > CYLMHD_LIMITER_MC) {
-            set_errmsg("batch: the batching form runs the minmod and MC limiters only");
-            return CYLMHD_EUNSUP;
-        }
-        if (p[d].source != CYLMHD_SOURCE_SPIRAL_COOLING) {
-            set_errmsg("batch: the batching form runs the default source (SpiralGravityCooling) only");
-            return CYLMHD_EUNSUP;
-        }
     }
     cylmhd_batch *b = new cylmhd_batch();
     b->n = n;
-    b->mc = p[0].limiter == CYLMHD_LIMITER_MC;
+    b->lim = p[0].limiter;
+    for (int d = 0; d < n; d++) b->alt = b->alt || p[d].source != CYLMHD_SOURCE_SPIRAL_COOLING;
     b->hll = p[0].solver == CYLMHD_SOLVER_HLL;
     b->rk3 = p[0].rk != 2;
     b->nst = b->rk3 ? 3 : 2;

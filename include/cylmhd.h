@@ -192,8 +192,7 @@ typedef struct cylmhd_batch cylmhd_batch;
  * HLLD, RK3, FP32 path.  p must hold n entries. */
 int cylmhd_params_batch(cylmhd_params *p, int n, uint64_t seed);
 /* All discs must share limiter, solver and RK order (else CYLMHD_EINVAL);
- * the limiter must be minmod or MC and the source the default
- * (CYLMHD_SOURCE_SPIRAL_COOLING): others give CYLMHD_EUNSUP. */
+ * each disc may run its own source function (p.source). */
 int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out);
 int cylmhd_batch_destroy(cylmhd_batch *b);
 /* The context of disc d (owned by the batch; do not destroy it). */
