@@ -61,7 +61,7 @@ __global__ __launch_bounds__((tile_threads<float, S>()), (tile_waves<float, S>()
     for (int it = b; it < nitems; it += gridDim.x) {
         const int4 w = items[it];
         if (!bd[w.x].alive) continue;
-        fused_tile<float, LIM, HLL, S, ALT>(descs[w.x], w.y, w.z, it);
+        fused_tile<float, LIM, HLL, S, ALT, true>(descs[w.x], w.y, w.z, it);
     }
 }
 

@@ -286,7 +286,7 @@ template <typename T, int SHAPE> constexpr int tile_waves() { return Tile<T, SHA
  * ibeg..iend of a TJ x TK (phi x z) column.  slot indexes the Setdt
  * partial the item writes on the last stage.  ALT: any source function
  * (Consts::source); false compiles SpiralGravityCooling only. */
-template <typename T, int LIM, bool HLL, int SHAPE, bool ALT>
+template <typename T, int LIM, bool HLL, int SHAPE, bool ALT, bool DEFER = false>
 __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg, int slot)
 {
     constexpr int TJ = Tile<T, SHAPE>::TJ, TK = Tile<T, SHAPE>::TK;
@@ -461,7 +461,24 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
      * barrier), and the halo of plane i goes into halo positions that only
      * the state pass of iteration i reads. */
     for (int i = ibeg; i <= iend; i++) {
-        if (i + 2 <= g.NX - 1 && i < g.NX - 2) load_center(i + 2);
+        /* the centre cells of plane i+2 go into the ring slot of plane i-1.
+         * DEFER: their loads are issued here and converted and written
+         * after the state pass (a thread's own ring column, read next by its
+         * own r-states after the face barrier and by the others two planes
+         * on), so that the latency overlaps the halo, the first barrier and
+         * the state pass; this pays where the extra live registers do not
+         * spill (the batch kernel), and not in the single-disc kernel */
+        const bool pre = i + 2 <= g.NX - 1 && i < g.NX - 2;
+        T raw[9];
+        if constexpr (DEFER) {
+            if (pre && is_cell) {
+                const size_t cp = SI(i + 2, jmap(g, gj), kmap(g, gk));
+#pragma unroll
+                for (int m = 0; m < 9; m++) raw[m] = a.Uin[m * NS + cp];
+            }
+        } else {
+            if (pre) load_center(i + 2);
+        }
         load_halo(i);
         stage_rows(i);
         if (MC && zedge) {
@@ -489,6 +506,16 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             cell_states<2, LIM>(c, vm, v0, vp, s_odz[zc + 1][0], s_odz[zc + 1][1], T(1.0), up, lo);
 #pragma unroll
             for (int m = 0; m < 9; m++) { sz[0][m * NZT + t] = up[m]; sz[1][m * NZT + t] = lo[m]; }
+        }
+        if constexpr (DEFER) {
+            if (pre && is_cell) {
+                T v[9];
+                raw[8] = T(c.psicoef) * raw[8];
+                Prec<T>::utov(c, raw, ro(a.itab, (size_t)(i + 2) * IT + IT_OX), v);
+                const int off = ((i + 2) % 3) * SLOT + cbase;
+#pragma unroll
+                for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
+            }
         }
         lds_barrier();
         /* ---- phi faces: (TJ+1) x TK ------------------------------------- */
