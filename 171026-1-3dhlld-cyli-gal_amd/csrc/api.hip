@@ -51,6 +51,7 @@ struct cylmhd_ctx {
     double *d_slice = nullptr;      /* one slice, the largest plane */
     size_t slice_cap = 0;
     bool have_state = false;
+    bool batch_layout = false;      /* a disc of a batch (z-side ghosts packed) */
     /* timing of the per-stage hot kernels */
     bool timing = false;
     hipEvent_t ev[kEventPairs][2];
@@ -223,6 +224,7 @@ int alloc_state(cylmhd_ctx *c)
         /* fused paths keep no V and no face arrays; V is only materialised
          * on demand (Setdt, download) into a scratch array */
         if ((rc = dalloc(c, &d.V, 9 * d.N))) return rc;
+        c->fs.zg = c->batch_layout;
         if ((rc = fused_alloc(c->fs, c->dg, d, c->p.path == CYLMHD_PATH_FUSED32,
                               [c](size_t bytes, void **ptr) {
                                   char *q;
@@ -278,6 +280,13 @@ const char *cylmhd_error_string(int code)
 
 int cylmhd_create(const cylmhd_params *p, cylmhd_ctx **out)
 {
+    return cyl::create_ctx(p, out, false);
+}
+
+}  // extern "C"
+
+int cyl::create_ctx(const cylmhd_params *p, cylmhd_ctx **out, bool batch_layout)
+{
     if (!out) return CYLMHD_EINVAL;
     *out = nullptr;
     g_errmsg[0] = 0;
@@ -291,6 +300,7 @@ int cylmhd_create(const cylmhd_params *p, cylmhd_ctx **out)
     }
     cylmhd_ctx *c = new cylmhd_ctx();
     c->p = *p;
+    c->batch_layout = batch_layout;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
         rc = hip_fail(e, "hipStreamCreate");
         destroy_ctx(c);
@@ -303,6 +313,8 @@ int cylmhd_create(const cylmhd_params *p, cylmhd_ctx **out)
     *out = c;
     return CYLMHD_OK;
 }
+
+extern "C" {
 
 int cylmhd_destroy(cylmhd_ctx *c)
 {
@@ -423,6 +435,10 @@ int cylmhd_gravity(cylmhd_ctx *c, double t)
 int cylmhd_step(cylmhd_ctx *c, double dt, double ch)
 {
     if (!c || !c->have_state) return CYLMHD_EINVAL;
+    if (c->batch_layout) {
+        snprintf(g_errmsg, sizeof g_errmsg, "this disc belongs to a batch: cylmhd_batch_run advances it");
+        return CYLMHD_EINVAL;
+    }
     const Consts k = make_consts(c, dt, ch);
     const bool rk3 = c->p.rk != 2;
     const int nst = rk3 ? 3 : 2;

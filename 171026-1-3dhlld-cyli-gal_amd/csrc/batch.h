@@ -21,6 +21,9 @@ struct DiscView {
 
 /* fills *v for a context created with CYLMHD_PATH_FUSED32 */
 int disc_view(cylmhd_ctx *c, DiscView *v);
+/* cylmhd_create; batch_layout: the state buffers keep their z-side ghost
+ * cells packed after the body (the batch kernels' layout) */
+int create_ctx(const cylmhd_params *p, cylmhd_ctx **out, bool batch_layout);
 /* the stage constants of main.c's step for this disc (dt, ch, psicoef) */
 Consts disc_consts(const cylmhd_ctx *c, double dt, double ch);
 /* the context's own Setdt partials no longer describe its state */

@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_halo_batch(const FArgs<float> *__restri
     const int d = find_disc(off, n, blockIdx.x);
     if (!bd[d].alive) return;
     const size_t nb = off[d + 1] - off[d];
-    halo_range<float>(descs[d], (size_t)(blockIdx.x - off[d]) * 256 + threadIdx.x, nb * 256);
+    halo_range<float, true>(descs[d], (size_t)(blockIdx.x - off[d]) * 256 + threadIdx.x, nb * 256);
 }
 
 /* Setdt partials of a stored state (before the first step): the blocks of
@@ -464,7 +464,7 @@ int backup(cylmhd_batch *b, bool save)
 {
     size_t off = 0;
     for (int d = 0; d < b->n; d++) {
-        const size_t cnt = 9 * b->view[d].fs->NS;
+        const size_t cnt = b->view[d].fs->nstate;
         float *un = (float *)b->view[d].fs->Un;
         if (save) BCHK(hipMemcpyAsync(b->d_backup + off, un, cnt * sizeof(float), hipMemcpyDeviceToDevice, b->stream));
         else BCHK(hipMemcpyAsync(un, b->d_backup + off, cnt * sizeof(float), hipMemcpyDeviceToDevice, b->stream));
@@ -499,13 +499,13 @@ int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out)
     size_t tot = 0;
     for (int d = 0; d < n && !rc; d++) {
         cylmhd_ctx *c = nullptr;
-        rc = cylmhd_create(&p[d], &c);
+        rc = create_ctx(&p[d], &c, true);           /* batch layout of the z-side ghosts */
         if (rc) break;
         b->disc.push_back(c);
         DiscView v;
         rc = disc_view(c, &v);
         b->view.push_back(v);
-        tot += 9 * v.fs->NS;
+        tot += v.fs->nstate;
     }
     if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) rc = CYLMHD_EHIP;
     if (!rc && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)) rc = CYLMHD_EHIP;

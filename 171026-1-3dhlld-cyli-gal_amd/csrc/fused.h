@@ -34,6 +34,11 @@ struct FusedState {
      * NZP cells with k = 1 at offset 16 (64-byte aligned FP32 tile rows) */
     size_t NS = 0;
     int NZP = 0, NZ = 0;
+    /* batch layout (discs of a batch only): the z-side ghost cells (k = 0,
+     * NZ-1) of each state buffer are not in its [9][NS] body but packed
+     * after it, [2][NX][NY][9]; nstate = elements per state buffer */
+    bool zg = false;
+    size_t nstate = 0;
     /* geometry packed for the stage kernel (few base pointers):
      *   itab [NX][IT] FP64  x, xm, oneox, oneodx, xabs, spiral r factors
      *   ikg  [NX*NZ][4] T   g1, g2, g3, g5

@@ -110,12 +110,23 @@ __device__ __forceinline__ size_t pad_index(int nz, int NZP, size_t r)
     return row * (size_t)NZP + (r - row * NZ) + KOFF;
 }
 
+/* element (m, cell r of the reference layout) of a state buffer: the padded
+ * body, or with the batch layout (zg) for the z-side ghosts the packed
+ * planes after it */
+__device__ __forceinline__ size_t state_index(const DevGeom &g, int NZP, size_t NS, bool zg, size_t m, size_t r)
+{
+    const size_t NZ = g.NZ, row = r / NZ, k = r - row * NZ;
+    if (zg && (k == 0 || k == NZ - 1)) return 9 * NS + ((k != 0 ? (size_t)g.NX * g.NY : 0) + row) * 9 + m;
+    return m * NS + row * (size_t)NZP + k + KOFF;
+}
+
 template <typename T>
-__global__ void k_to_state(DevGeom g, int NZP, size_t NS, const double *__restrict__ src, T *__restrict__ dst, size_t N)
+__global__ void k_to_state(DevGeom g, int NZP, size_t NS, bool zg, const double *__restrict__ src, T *__restrict__ dst,
+                           size_t N)
 {
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < 9 * N; q += (size_t)gridDim.x * blockDim.x) {
         const size_t m = q / N, r = q - m * N;
-        dst[m * NS + pad_index(g.NZ, NZP, r)] = T(src[q]);
+        dst[state_index(g, NZP, NS, zg, m, r)] = T(src[q]);
     }
 }
 
@@ -127,15 +138,14 @@ __global__ void k_rhofloor(int NZ, int NZP, const double *__restrict__ Uinit0, T
 }
 
 template <typename T>
-__global__ void k_export(DevGeom g, int NZP, size_t NS, Consts c, const T *__restrict__ U, double *__restrict__ U64,
-                         double *__restrict__ V64, size_t N)
+__global__ void k_export(DevGeom g, int NZP, size_t NS, bool zg, Consts c, const T *__restrict__ U,
+                         double *__restrict__ U64, double *__restrict__ V64, size_t N)
 {
     const size_t NYZ = (size_t)g.NY * g.NZ;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < N; q += (size_t)gridDim.x * blockDim.x) {
-        const size_t qs = pad_index(g.NZ, NZP, q);
         double u[9], v[9];
 #pragma unroll
-        for (int m = 0; m < 9; m++) { u[m] = (double)U[m * NS + qs]; U64[m * N + q] = u[m]; }
+        for (int m = 0; m < 9; m++) { u[m] = (double)U[state_index(g, NZP, NS, zg, m, q)]; U64[m * N + q] = u[m]; }
         utov<double>(c, u, g.oneox[q / NYZ], v);
 #pragma unroll
         for (int m = 0; m < 9; m++) V64[m * N + q] = v[m];
@@ -143,7 +153,7 @@ __global__ void k_export(DevGeom g, int NZP, size_t NS, Consts c, const T *__res
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag_t(DevGeom g, int NZP, const T *__restrict__ U, size_t N,
+__global__ __launch_bounds__(256) void k_diag_t(DevGeom g, int NZP, bool zg, const T *__restrict__ U, size_t N,
                                                 double *__restrict__ red)
 {
     /* N is the component stride of the padded state arrays */
@@ -157,7 +167,10 @@ __global__ __launch_bounds__(256) void k_diag_t(DevGeom g, int NZP, const T *__r
         const double b4 = U[4 * N + cc], b5 = U[5 * N + cc], b6 = U[6 * N + cc];
         const double brp = 0.5 * (b4 + (double)U[4 * N + cc + si]), brm = 0.5 * (b4 + (double)U[4 * N + cc - si]);
         const double bpp = 0.5 * (b5 + (double)U[5 * N + cc + sj]), bpm = 0.5 * (b5 + (double)U[5 * N + cc - sj]);
-        const double bzp = 0.5 * (b6 + (double)U[6 * N + cc + 1]), bzm = 0.5 * (b6 + (double)U[6 * N + cc - 1]);
+        /* the z neighbours k = 0 and NZ-1 are ghosts: packed planes with zg */
+        const size_t zp = zg && k == g.NZ - 2 ? 9 * N + (((size_t)g.NX + i) * g.NY + j) * 9 + 6 : 6 * N + cc + 1;
+        const size_t zm = zg && k == 1 ? 9 * N + ((size_t)i * g.NY + j) * 9 + 6 : 6 * N + cc - 1;
+        const double bzp = 0.5 * (b6 + (double)U[zp]), bzm = 0.5 * (b6 + (double)U[zm]);
         const double db = (g.g2[ik] * brp - g.g1[ik] * brm) + (g.g3[ik] * bpp - g.g3[ik] * bpm) + (g.g5[ik] * bzp - g.g5[ik] * bzm);
         acc[0] += (double)U[cc] * dv;
         acc[1] += (double)U[7 * N + cc] * dv;
@@ -215,8 +228,9 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     fs.NS = (size_t)g.NX * g.NY * fs.NZP;
     const size_t NS = fs.NS;
     int rc;
-    if ((rc = alloc(9 * NS * fs.esize, &fs.Un)) || (rc = alloc(9 * NS * fs.esize, &fs.A)) ||
-        (rc = alloc(9 * NS * fs.esize, &fs.B)) || (rc = alloc((9 * NS + 27 * (size_t)g.NX * g.NY) * fs.esize, &fs.shadow)) ||
+    fs.nstate = 9 * NS + (fs.zg ? 18 * (size_t)g.NX * g.NY : 0);
+    if ((rc = alloc(fs.nstate * fs.esize, &fs.Un)) || (rc = alloc(fs.nstate * fs.esize, &fs.A)) ||
+        (rc = alloc(fs.nstate * fs.esize, &fs.B)) || (rc = alloc((9 * NS + 27 * (size_t)g.NX * g.NY) * fs.esize, &fs.shadow)) ||
         (rc = alloc(NS * fs.esize, &fs.rhofloor)) || (rc = alloc(2 * 9 * (size_t)g.NY * g.NZ * fs.esize, &fs.edge)))
         return rc;
     void *p;
@@ -308,14 +322,14 @@ int load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &
 {
     (void)g;
     const size_t n = 9 * d.N;
-    hipLaunchKernelGGL(k_to_state<FT>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, U64, (FT *)fs.Un, d.N);
+    hipLaunchKernelGGL(k_to_state<FT>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, U64, (FT *)fs.Un, d.N);
     fs.dt_valid = false;
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
 int export_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c, double *V64)
 {
-    hipLaunchKernelGGL(k_export<FT>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, c, (const FT *)fs.Un, fs.u64_scratch, V64, d.N);
+    hipLaunchKernelGGL(k_export<FT>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, c, (const FT *)fs.Un, fs.u64_scratch, V64, d.N);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
@@ -413,6 +427,7 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
 int stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
           const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1)
 {
+    if (fs.zg) return CYLMHD_EINVAL;   /* a batch's disc: the batch advances it */
     /* one kernel per limiter x solver; the default source (SpiralGravityCooling)
      * runs a kernel compiled for it alone */
 #define CYL_STAGE(L, H, A) stage_t<FT, L, H, A>(s, fs, g, d, c, a, ev0, ev1)
@@ -437,7 +452,7 @@ int diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d)
 {
     const size_t M = (size_t)(g.NX - 2) * (g.NY - 2) * (g.NZ - 2);
     const int nb = (int)nblocks(M, 256, (unsigned)(d.nred / 4));
-    hipLaunchKernelGGL(k_diag_t<FT>, dim3(nb), dim3(256), 0, s, g, fs.NZP, (const FT *)fs.Un, fs.NS, d.red);
+    hipLaunchKernelGGL(k_diag_t<FT>, dim3(nb), dim3(256), 0, s, g, fs.NZP, fs.zg, (const FT *)fs.Un, fs.NS, d.red);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, d.red, nb);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }

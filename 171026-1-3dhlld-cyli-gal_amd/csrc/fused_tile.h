@@ -54,12 +54,19 @@ namespace FUSED_NS {
 
 using FT = FUSED_FT;
 
+constexpr int KOFF = 15;     /* k = 1 at offset 16 of a padded row */
+
 /* C3: cell index in the reference layout (Uinitial, the dt argmax);
  * SI: cell index in the state arrays of this path, whose k rows are padded
  * so that k = 1 (the first interior cell) starts a 64-byte line: a tile row
  * of 16 FP32 cells is then one aligned 64-byte access. */
 #define C3(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * g.NZ + (size_t)(k))
 #define SI(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * (size_t)a.NZP + (size_t)(k) + KOFF)
+/* ZG: with the batch layout the z-side ghost cells (k = 0: side 0, k = NZ-1:
+ * side 1) of a state buffer live after its [9][NS] body, [2][NX][NY][9], the
+ * nine values of a cell together (component m at ZG(...) + m), so that the
+ * ghost pass writes them to consecutive addresses */
+#define ZG(side, i, j) (9 * a.NS + ((((size_t)(side)) * g.NX + (size_t)(i)) * g.NY + (size_t)(j)) * 9)
 
 
 /* Loads from tables that no launch writes (geometry rows, packed source
@@ -174,6 +181,15 @@ __device__ __forceinline__ int kmap(const DevGeom &g, int kg) { return kg < 0 ? 
 
 /* stage input of one cell as primitives: psi damped (GLMSourceSolver) then UtoV */
 template <typename T>
+__device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, T *v, size_t stride)
+{
+    T u[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) u[m] = a.Uin[m * stride + cc];
+    u[8] = T(a.c.psicoef) * u[8];
+    Prec<T>::utov(a.c, u, ro(a.itab, (size_t)i * IT + IT_OX), v);
+}
+template <typename T>
 __device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, T *v)
 {
     T u[9];
@@ -181,6 +197,14 @@ __device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, T *v
     for (int m = 0; m < 9; m++) u[m] = a.Uin[m * a.NS + cc];
     u[8] = T(a.c.psicoef) * u[8];
     Prec<T>::utov(a.c, u, ro(a.itab, (size_t)i * IT + IT_OX), v);
+}
+/* cell (i, j, k) of any k; ZGL: the batch layout's z-side ghosts */
+template <bool ZGL, typename T>
+__device__ __forceinline__ void cell_V_at(const FArgs<T> &a, int i, int j, int k, T *v)
+{
+    const DevGeom &g = a.g;
+    if (ZGL && (k == 0 || k == g.NZ - 1)) cell_V(a, ZG(k != 0, i, j), i, v, 1);
+    else cell_V(a, SI(i, j, k), i, v);
 }
 
 /* Work-group barrier for LDS traffic only.  __syncthreads() fences global
@@ -271,7 +295,6 @@ __device__ __forceinline__ bool gravity_staged(const DevGeom &g, int i, cdp it, 
  * side): FP32 14 x 16 / 256 threads, FP64 6 x 16 / 128 threads.  The LDS
  * of one block (3-plane ring + phi and z state buffers) is 75 KB (FP32) or
  * 77 KB (FP64), so two blocks share a CU. */
-constexpr int KOFF = 15;     /* k = 1 at offset 16 of a padded row */
 
 template <typename T, int SHAPE> struct Tile;
 template <> struct Tile<float, 0>  { static constexpr int TJ = 14, TK = 16, WAVES = 2; };
@@ -286,7 +309,7 @@ template <typename T, int SHAPE> constexpr int tile_waves() { return Tile<T, SHA
  * ibeg..iend of a TJ x TK (phi x z) column.  slot indexes the Setdt
  * partial the item writes on the last stage.  ALT: any source function
  * (Consts::source); false compiles SpiralGravityCooling only. */
-template <typename T, int LIM, bool HLL, int SHAPE, bool ALT, bool DEFER = false>
+template <typename T, int LIM, bool HLL, int SHAPE, bool ALT, bool BATCH = false>
 __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg, int slot)
 {
     constexpr int TJ = Tile<T, SHAPE>::TJ, TK = Tile<T, SHAPE>::TK;
@@ -377,7 +400,8 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
     auto load_center = [&](int p) {
         if (is_cell && p >= 0 && p <= g.NX - 1) {
             T v[9];
-            cell_V(a, SI(p, jmap(g, gj), kmap(g, gk)), p, v);
+            if constexpr (BATCH) cell_V_at<true>(a, p, jmap(g, gj), kmap(g, gk), v);
+            else cell_V(a, SI(p, jmap(g, gj), kmap(g, gk)), p, v);
             const int off = (p % 3) * SLOT + cbase;
 #pragma unroll
             for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
@@ -393,7 +417,8 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
     auto load_halo = [&](int p) {
         if (t < NHALO) {
             T v[9];
-            cell_V(a, (size_t)p * g.NY * a.NZP + hcol, p, v);
+            if constexpr (BATCH) cell_V_at<true>(a, p, jmap(g, j0 + hj), kmap(g, k0 + hk), v);
+            else cell_V(a, (size_t)p * g.NY * a.NZP + hcol, p, v);
             const int off = (p % 3) * SLOT + hoff;
 #pragma unroll
             for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
@@ -462,7 +487,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
      * the state pass of iteration i reads. */
     for (int i = ibeg; i <= iend; i++) {
         /* the centre cells of plane i+2 go into the ring slot of plane i-1.
-         * DEFER: their loads are issued here and converted and written
+         * BATCH: their loads are issued here and converted and written
          * after the state pass (a thread's own ring column, read next by its
          * own r-states after the face barrier and by the others two planes
          * on), so that the latency overlaps the halo, the first barrier and
@@ -470,11 +495,14 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
          * spill (the batch kernel), and not in the single-disc kernel */
         const bool pre = i + 2 <= g.NX - 1 && i < g.NX - 2;
         T raw[9];
-        if constexpr (DEFER) {
+        if constexpr (BATCH) {
             if (pre && is_cell) {
-                const size_t cp = SI(i + 2, jmap(g, gj), kmap(g, gk));
+                const int kk = kmap(g, gk);
+                const bool zg = kk == g.NZ - 1;        /* past the end of a partial tile */
+                const size_t cp = zg ? ZG(1, i + 2, jmap(g, gj)) : SI(i + 2, jmap(g, gj), kk);
+                const size_t st = zg ? (size_t)1 : NS;
 #pragma unroll
-                for (int m = 0; m < 9; m++) raw[m] = a.Uin[m * NS + cp];
+                for (int m = 0; m < 9; m++) raw[m] = a.Uin[m * st + cp];
             }
         } else {
             if (pre) load_center(i + 2);
@@ -507,7 +535,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
 #pragma unroll
             for (int m = 0; m < 9; m++) { sz[0][m * NZT + t] = up[m]; sz[1][m * NZT + t] = lo[m]; }
         }
-        if constexpr (DEFER) {
+        if constexpr (BATCH) {
             if (pre && is_cell) {
                 T v[9];
                 raw[8] = T(c.psicoef) * raw[8];
@@ -558,7 +586,9 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     /* ZVlrPeriodicBoundary: the upper state of cell NZ-2 */
                     T vm[9], v0[9], vp[9];
                     const int jj = jmap(g, j0 + fzj), L = g.NZ - 2;
-                    cell_V(a, SI(i, jj, L - 1), i, vm); cell_V(a, SI(i, jj, L), i, v0); cell_V(a, SI(i, jj, L + 1), i, vp);
+                    cell_V(a, SI(i, jj, L - 1), i, vm); cell_V(a, SI(i, jj, L), i, v0);
+                    if constexpr (BATCH) cell_V_at<true>(a, i, jj, L + 1, vp);
+                    else cell_V(a, SI(i, jj, L + 1), i, vp);
                     cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), vl, (T *)nullptr);
                 } else {
                     /* ZLeftVlVzAntisymBzsym: lower state of cell 2, vz negated */
@@ -581,7 +611,9 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     /* ZVlrPeriodicBoundary: the lower state of cell 1 */
                     T vm[9], v0[9], vp[9];
                     const int jj = jmap(g, j0 + fzj);
-                    cell_V(a, SI(i, jj, 0), i, vm); cell_V(a, SI(i, jj, 1), i, v0); cell_V(a, SI(i, jj, 2), i, vp);
+                    if constexpr (BATCH) cell_V_at<true>(a, i, jj, 0, vm);
+                    else cell_V(a, SI(i, jj, 0), i, vm);
+                    cell_V(a, SI(i, jj, 1), i, v0); cell_V(a, SI(i, jj, 2), i, vp);
                     cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), (T *)nullptr, vr);
                 } else {
                     /* ZRightVrFreeBoundary (minmod) / ZVlrFreeBoundary (vanLeer): Vr = Vl */
@@ -810,7 +842,7 @@ __device__ __forceinline__ size_t halo_cells(const DevGeom &g)
 
 /* ghost cells first, first + stride, ... of the composed boundary map;
  * every lane of the wave must call it (the counters are wave-reduced) */
-template <typename T>
+template <typename T, bool ZGL = false>
 __device__ __forceinline__ void halo_range(const FArgs<T> &a, size_t first, size_t stride)
 {
     const DevGeom &g = a.g;
@@ -862,7 +894,11 @@ __device__ __forceinline__ void halo_range(const FArgs<T> &a, size_t first, size
          * row padding: KOFF cells before k = 1, NZP - NZ - KOFF after NZ-1),
          * so the whole line is written, avoiding a partial-line write. */
         constexpr int E = 64 / (int)sizeof(T);
-        if ((k == 0 && (dst % E) == E - 1) || (k == (int)NZ - 1 && (dst % E) == 0)) {
+        if (ZGL && (k == 0 || k == (int)NZ - 1)) {
+            T *q = a.Uout + ZG(k != 0, i, j);        /* the batch layout's packed slot */
+#pragma unroll
+            for (int m = 0; m < 9; m++) q[m] = u[m];
+        } else if ((k == 0 && (dst % E) == E - 1) || (k == (int)NZ - 1 && (dst % E) == 0)) {
             const size_t line = dst - dst % E;
 #pragma unroll
             for (int m = 0; m < 9; m++) {

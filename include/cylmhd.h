@@ -195,7 +195,9 @@ int cylmhd_params_batch(cylmhd_params *p, int n, uint64_t seed);
  * each disc may run its own source function (p.source). */
 int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out);
 int cylmhd_batch_destroy(cylmhd_batch *b);
-/* The context of disc d (owned by the batch; do not destroy it). */
+/* The context of disc d (owned by the batch; do not destroy it).  Set-up,
+ * dumps, slices and diagnostics work on it; cylmhd_step does not (the batch
+ * keeps its own state layout): cylmhd_batch_run advances it. */
 int cylmhd_batch_disc(cylmhd_batch *b, int d, cylmhd_ctx **ctx);
 /* Autotuner: times `steps` steps for each tile shape x r-segment length x
  * launch grid on the current states, restores them, keeps the fastest.

@@ -147,3 +147,16 @@ def test_batch_rejects_mixed_limiters(gpu):
     with pytest.raises(cm.CylmhdError) as e:
         cm.Batch(ps)
     assert e.value.code == cm.EINVAL
+
+
+def test_batch_disc_is_not_stepped_alone(gpu):
+    """A batch's disc keeps the batch's state layout; only the batch steps it."""
+    ps = batch_params()
+    with start_batch(ps) as b:
+        g = b.disc(0)
+        dt, ch, _ = g.setdt()
+        with pytest.raises(cm.CylmhdError) as e:
+            g.step(dt, ch)
+        assert e.value.code == cm.EINVAL
+        U, V = g.download()                  # set-up, dumps and diagnostics still work
+        assert np.isfinite(U).all() and g.diagnostics()["mass"] > 0
