@@ -32,7 +32,8 @@ extern "C" {
 #define CYLMHD_ENOMEM     -2   /* device or host allocation failed         */
 #define CYLMHD_EHIP       -3   /* a HIP runtime call failed                */
 #define CYLMHD_EIO        -4   /* dump / restart file could not be opened  */
-#define CYLMHD_EUNSUP     -5   /* option not available on this path        */
+#define CYLMHD_EUNSUP     -5   /* option not available (reserved: every current
+                                  option runs on every path)                  */
 
 /* ---- scheme selectors (the reference picks these by commenting calls) -- */
 /* limiter: firststep.h:37-38 (MinmodLimiterX is the active one)          */
@@ -268,7 +269,8 @@ typedef struct cylmhd_idl_planes {
 /* The text slices the reference's IDL scripts read, for dump number nout:
  * <root>/idl<nout>/ (and <root>/idl0/) r.txt, phi.txt, z.txt, x.txt, and
  * <root>/idl<nout>/<name>-<nout>.dat for the names the scripts open, each
- * in the order its fltarr + readf expects.  planes may be NULL (defaults). */
+ * in the order its fltarr + readf expects.  planes may be NULL (defaults).
+ * jy (NY-2) must be even: the xz cuts pair each phi with phi + pi. */
 int cylmhd_idl_output(cylmhd_ctx *c, const char *root, int nout, const cylmhd_idl_planes *planes);
 
 #ifdef __cplusplus
