@@ -638,8 +638,14 @@ int cylmhd_idl_output(cylmhd_ctx *c, const char *root, int nout, const cylmhd_id
     if (!c || !root || !c->have_state || nout < 0) return CYLMHD_EINVAL;
     const cyl_grid &g = c->g;
     const int NX = g.NX, NY = g.NY, NZ = g.NZ;
-    cylmhd_idl_planes o = {1, 100, 1};
+    cylmhd_idl_planes o = {1, 100, 1, {0, 0, 0, 0}, 0};
     if (planes) o = *planes;
+    for (int K = 0; K < 4; K++) {
+        if (o.azim_i[K] == 0) o.azim_i[K] = 1 + (K + 1) * (NX - 2) / 5;
+        if (o.azim_i[K] < 1 || o.azim_i[K] > NX - 2) return CYLMHD_EINVAL;
+    }
+    if (o.azim_k == 0) o.azim_k = 1;
+    if (o.azim_k < 1 || o.azim_k > NZ - 2) return CYLMHD_EINVAL;
     if (o.k100 > NZ - 2) o.k100 = NZ - 2;
     if (o.kplane < 1 || o.kplane > NZ - 2 || o.k100 < 1 || o.jplane < 1 || o.jplane > NY - 2 || (NY - 2) % 2)
         return CYLMHD_EINVAL;
@@ -701,6 +707,24 @@ int cylmhd_idl_output(cylmhd_ctx *c, const char *root, int nout, const cylmhd_id
     for (const Item &it : items) {
         snprintf(name, sizeof name, "%s-%d.dat", it.name, nout);
         if ((rc = slice_file(c, buf, dir, name, it.q, it.plane, it.a))) return rc;
+    }
+    /* azimuthal profiles, fltarr(ny) + readf: phiout0..3.pro, fourieramp.pro */
+    static const struct { const char *name; int q; } prof[] = {
+        {"potential", CYLMHD_Q_SPIRALPOT}, {"rho", 0}, {"te", CYLMHD_Q_TEMP},
+        {"vperp", CYLMHD_Q_VPERP}, {"vpara", CYLMHD_Q_VPARA},
+        {"velocitypara", CYLMHD_Q_VPARA}};   /* phiout1..3.pro read the same profile under this name */
+    const size_t nph = (size_t)NY - 2;
+    buf.resize(nph);
+    for (int K = 0; K < 4; K++) {
+        char adir[4096];
+        if (snprintf(adir, sizeof adir, "%s/azimuth-%d", root, K) >= (int)sizeof adir) return CYLMHD_EINVAL;
+        if ((rc = cyl_mkdir(adir)) || (rc = cyl_write_text(adir, "phi.dat", g.y + 1, nph))) return rc;
+        for (const auto &pr : prof) {
+            snprintf(name, sizeof name, "%s-%d.dat", pr.name, nout);
+            if ((rc = slice_host(c, pr.q, CYLMHD_PLANE_PHI, o.azim_i[K], o.azim_k, buf.data(), nph)) ||
+                (rc = cyl_write_text(adir, name, buf.data(), nph)))
+                return rc;
+        }
     }
     return CYLMHD_OK;
 }

@@ -49,6 +49,22 @@ __device__ double slice_value(const DevGeom &g, const Consts &c, const double *_
     }
     case CYLMHD_Q_RHOBZ:
         return rho * b6;
+    case CYLMHD_Q_SPIRALPOT: {
+        /* E cos(phase) with E / r = the amplitude of the arm's phi force
+         * eps m1 q s(z) (x/sr^2) (1+x^2/sr^2)^-1.5 m sin(phase)
+         * (spiralgravity.h:93-98, m1 = 10): F_phi = -(1/r) dPhi/dphi */
+        const size_t ik = (size_t)i * g.NZ + k;
+        const double cosphase = g.sincos[(size_t)g.NX * g.NY + (size_t)i * g.NY + j];
+        return g.epssp * 10.0 * g.sp_q[ik] * g.sp_szs[k] * g.sp_x2s2o15[i] * cosphase;
+    }
+    case CYLMHD_Q_VPARA:
+    case CYLMHD_Q_VPERP: {
+        /* arm direction: phase = m (phi + cot(p) ln r) + const is constant
+         * along (sin p, -cos p) in (r, phi); oriented with the rotation */
+        const double sp = 1.0 / sqrt(1.0 + g.cotpitch * g.cotpitch), cp = g.cotpitch * sp;
+        const double vr = V[1 * N + cc], vp = V[2 * N + cc];
+        return q == CYLMHD_Q_VPARA ? vp * cp - vr * sp : vr * cp + vp * sp;
+    }
     }
     return 0.0;
 }

@@ -24,8 +24,8 @@ GRAV_GALACTIC, GRAV_POINTMASS = 0, 1
 PATH_PLAIN64, PATH_FUSED64, PATH_FUSED32 = 0, 1, 2
 # slice quantities beyond the primitives 0..8 and slice planes (include/cylmhd.h)
 Q_RHO, Q_VR, Q_VPHI, Q_VZ, Q_BR, Q_BPHI, Q_BZ, Q_PR, Q_PSI = range(9)
-Q_BENERGY, Q_BETA, Q_TEMP, Q_ONEOTCOOL, Q_VDOTB, Q_DIVB, Q_RHOBZ = range(9, 16)
-NQ = 16
+Q_BENERGY, Q_BETA, Q_TEMP, Q_ONEOTCOOL, Q_VDOTB, Q_DIVB, Q_RHOBZ, Q_SPIRALPOT, Q_VPARA, Q_VPERP = range(9, 19)
+NQ = 19
 (PLANE_RPHI, PLANE_RPHI_ZMEAN, PLANE_RPHI_ZINT, PLANE_RZ, PLANE_ZR, PLANE_XZ,
  PLANE_XZ_PHIMEAN, PLANE_PHI) = range(8)
 OK, EINVAL, ENOMEM, EHIP, EIO, EUNSUP = 0, -1, -2, -3, -4, -5
@@ -77,7 +77,8 @@ class Diag(C.Structure):
 
 
 class IdlPlanes(C.Structure):
-    _fields_ = [("kplane", C.c_int), ("k100", C.c_int), ("jplane", C.c_int)]
+    _fields_ = [("kplane", C.c_int), ("k100", C.c_int), ("jplane", C.c_int), ("azim_i", C.c_int * 4),
+                ("azim_k", C.c_int)]
 
 
 class CylmhdError(RuntimeError):
@@ -289,9 +290,10 @@ class Context:
     def gdl_output(self, dirname, nout):
         check(lib().cylmhd_gdl_output_ctx(self.h, dirname.encode(), nout), "gdl_output_ctx")
 
-    def idl_output(self, root, nout, kplane=1, k100=100, jplane=1):
-        """The text slices the reference's IDL scripts read (idl<nout>/...)."""
-        pl = IdlPlanes(kplane, k100, jplane)
+    def idl_output(self, root, nout, kplane=1, k100=100, jplane=1, azim_i=(0, 0, 0, 0), azim_k=0):
+        """The text slices the reference's IDL scripts read (idl<nout>/...,
+        azimuth-K/...)."""
+        pl = IdlPlanes(kplane, k100, jplane, (C.c_int * 4)(*azim_i), azim_k)
         check(lib().cylmhd_idl_output(self.h, root.encode(), nout, C.byref(pl)), "idl_output")
 
     def setdt(self):

@@ -16,7 +16,8 @@
  * GalacticDisc3.  With every dump: --gdl also writes GDLOutputCylinder's
  * text files (output.h:280-384; main.c:371-372 has the call commented) into
  * DIR, --idl the text slices the reference's IDL scripts read into
- * ROOT/idl<nout>/.  --convert FIRST LAST runs no step: it reads the dumps
+ * ROOT/idl<nout>/ and ROOT/azimuth-K/ (the arm potential at the dump time;
+ * with --convert at nout * tint, the restart's time, main.c:269).  --convert FIRST LAST runs no step: it reads the dumps
  * DIR/FIRST .. DIR/LAST (their grid must be the one --config/--grid gives)
  * and writes their text files.  Exit status: 0, or 1 on the dt guard / any
  * library error.
@@ -75,12 +76,15 @@ static void galactic_u_bcs(int NX, int NY, int NZ, double *U)
             }
 }
 
-/* the text files of dump nout from the context's current state */
-static int text_output(cylmhd_ctx *ctx, int gdl, const char *idl, const char *outdir, int nout)
+/* the text files of dump nout from the context's current state at time t
+ * (the arm potential of the azimuthal profiles is SpiralGravity's at t; the
+ * next step sets it again at the same t) */
+static int text_output(cylmhd_ctx *ctx, int gdl, const char *idl, const char *outdir, int nout, double t)
 {
     int rc;
     if (gdl && (rc = cylmhd_gdl_output_ctx(ctx, outdir, nout))) return fail("gdl_output", rc);
-    if (idl && (rc = cylmhd_idl_output(ctx, idl, nout, NULL))) return fail("idl_output", rc);
+    if (idl && ((rc = cylmhd_gravity(ctx, t)) || (rc = cylmhd_idl_output(ctx, idl, nout, NULL))))
+        return fail("idl_output", rc);
     return 0;
 }
 
@@ -145,7 +149,7 @@ int main(int argc, char **argv)
             if ((rc = cylmhd_restart_read(dir, NX, NY, NZ, V))) return fail(dir, rc);
             if ((rc = cylmhd_vtou_host(&p, V, U))) return fail("vtou", rc);
             if ((rc = cylmhd_set_initial(ctx, U, V))) return fail("set_initial", rc);
-            if (text_output(ctx, gdl, idl, outdir, n)) return 1;
+            if (text_output(ctx, gdl, idl, outdir, n, n * tint)) return 1;   /* main.c:269 */
             printf("converted %s\n", dir);
         }
         cylmhd_destroy(ctx);
@@ -176,7 +180,7 @@ int main(int argc, char **argv)
         if ((rc = cylmhd_set_initial(ctx, U, V))) return fail("set_initial", rc);
         dirname_of(dir, sizeof dir, outdir, nout);
         if ((rc = cylmhd_binary_output(dir, NX, NY, NZ, t, x, y, z, V))) return fail(dir, rc);
-        if (text_output(ctx, gdl, idl, outdir, nout)) return 1;
+        if (text_output(ctx, gdl, idl, outdir, nout, t)) return 1;
     }
     printf("complete model setting.\n");
 
@@ -217,7 +221,7 @@ int main(int argc, char **argv)
             if ((rc = cylmhd_download_state(ctx, NULL, V))) return fail("download", rc);
             dirname_of(dir, sizeof dir, outdir, nout);
             if ((rc = cylmhd_binary_output(dir, NX, NY, NZ, t, x, y, z, V))) return fail(dir, rc);
-            if (text_output(ctx, gdl, idl, outdir, nout)) return 1;
+            if (text_output(ctx, gdl, idl, outdir, nout, t)) return 1;
         }
         if (t > tstop) {                                      /* main.c:383-390 */
             printf("time is over %.3e.\n", tstop);

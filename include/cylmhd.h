@@ -235,7 +235,14 @@ int cylmhd_restart_read(const char *dir, int nx, int ny, int nz, double *V);
 #define CYLMHD_Q_VDOTB     13   /* v . B                                               */
 #define CYLMHD_Q_DIVB      14   /* cell div B as cylmhd_diagnostics (0 on ghost cells) */
 #define CYLMHD_Q_RHOBZ     15   /* rho Bz, the Faraday-depth integrand                 */
-#define CYLMHD_NQ          16
+#define CYLMHD_Q_SPIRALPOT 16   /* arm potential eps m1 q s(z) x^2/sr^2 (1+x^2/sr^2)^-1.5
+                                   cos(phase) of the last cylmhd_gravity call: the
+                                   potential whose -(1/r) d/dphi is the arm's phi force
+                                   (spiralgravity.h:93-98)                              */
+#define CYLMHD_Q_VPARA     17   /* velocity along the arm: vphi cos(p) - vr sin(p), p the
+                                   pitch angle (the arms are phase = const, input.h:96) */
+#define CYLMHD_Q_VPERP     18   /* velocity across the arm: vr cos(p) + vphi sin(p)    */
+#define CYLMHD_NQ          19
 /* plane (a, b are indices with the ghosts counted, as the reference's) */
 #define CYLMHD_PLANE_RPHI        0 /* k = a: i outer, j inner, interior i, j (output.h:318-330) */
 #define CYLMHD_PLANE_RPHI_ZMEAN  1 /* mean over the interior k, same order             */
@@ -265,11 +272,15 @@ typedef struct cylmhd_idl_planes {
     int kplane;   /* k of the z-plane slices (zplane*, divb0, vdotb, temp-rphi); 1 */
     int k100;     /* k of z100bphi; 100, clipped to NZ-2                              */
     int jplane;   /* j of the rz cuts and of the xz cut through the axis; 1          */
+    int azim_i[4];/* i of the azimuthal profiles azimuth-0..3; 0: 1 + (K+1)(NX-2)/5  */
+    int azim_k;   /* k of the azimuthal profiles; 0: 1                                */
 } cylmhd_idl_planes;
 /* The text slices the reference's IDL scripts read, for dump number nout:
  * <root>/idl<nout>/ (and <root>/idl0/) r.txt, phi.txt, z.txt, x.txt, and
  * <root>/idl<nout>/<name>-<nout>.dat for the names the scripts open, each
- * in the order its fltarr + readf expects.  planes may be NULL (defaults).
+ * in the order its fltarr + readf expects; <root>/azimuth-K/ (K = 0..3)
+ * phi.dat and {potential,rho,te,vperp,vpara}-<nout>.dat, azimuthal profiles
+ * at (azim_i[K], azim_k).  planes may be NULL (defaults).
  * jy (NY-2) must be even: the xz cuts pair each phi with phi + pi. */
 int cylmhd_idl_output(cylmhd_ctx *c, const char *root, int nout, const cylmhd_idl_planes *planes);
 

@@ -15,6 +15,10 @@ has one):
   DIVB     finite-volume divergence with face means and g1, g2, g3, g5
            (cylindricalgrid.h:114-125), 0 outside the interior
   RHOBZ    rho b6
+  SPIRALPOT eps m1 q s(z) x^2/sr^2 (1+x^2/sr^2)^-1.5 cos(phase): the potential
+           whose -(1/r) d/dphi is the arm's phi force (spiralgravity.h:37-98)
+  VPARA, VPERP  velocity along / across the arms (phase = const), pitch p:
+           vphi cos p - vr sin p,  vr cos p + vphi sin p
 Planes (file order):
   RPHI     k = a, i outer, j inner, interior      output.h:318-330
   RZ       j = a, k outer, i inner, interior      output.h:366-380
@@ -29,14 +33,36 @@ import os
 
 import numpy as np
 
-Q_BENERGY, Q_BETA, Q_TEMP, Q_ONEOTCOOL, Q_VDOTB, Q_DIVB, Q_RHOBZ = range(9, 16)
+Q_BENERGY, Q_BETA, Q_TEMP, Q_ONEOTCOOL, Q_VDOTB, Q_DIVB, Q_RHOBZ, Q_SPIRALPOT, Q_VPARA, Q_VPERP = range(9, 19)
+
+
+def spiral_tables(p, x, y, z, t):
+    """The arm factors of SpiralGravity (spiralgravity.h:37-77) at time t:
+    q[i,k] = 1/sqrt(x^2 + (a + sqrt(z^2 + b^2))^2), s[k] = scalez/sqrt(z^2 +
+    scalez^2), x2s2o15[i] = x^2/sr^2 (1 + x^2/sr^2)^-1.5, cos(phase)[i,j] with
+    phase = m (y - (Omega_p - Omega_sp) t + cot(pitch) ln(x/scalephase)) and
+    eps(t) (spiralgravity.h:41-42)."""
+    a1, b1 = 0.65, 0.026
+    xs = x * (1.0 / p.scaler)
+    x2s2 = xs * xs
+    x2s2o15 = x2s2 * (1.0 / (1.0 + x2s2)) ** 1.5
+    q = 1.0 / np.sqrt((x * x)[:, None] + ((a1 + np.sqrt(z * z + b1 * b1)) * (a1 + np.sqrt(z * z + b1 * b1)))[None, :])
+    szs = p.scalez / np.sqrt(z * z + p.scalez * p.scalez)
+    cot = 1.0 / np.tan(p.pitch)
+    phase = p.armnum * (y[None, :] - (p.omegapattern - p.omegasp) * t + cot * np.log(x * (1.0 / p.scalephase))[:, None])
+    sw = (t - p.tausp) / abs(t - p.tausp)
+    eps = 0.5 * p.epssp0 * (t / p.tausp * (1.0 - sw) + (1.0 + sw))
+    return dict(q=q, szs=szs, x2s2o15=x2s2o15, cosphase=np.cos(phase), eps=eps, cot=cot)
+
+
 (PLANE_RPHI, PLANE_RPHI_ZMEAN, PLANE_RPHI_ZINT, PLANE_RZ, PLANE_ZR, PLANE_XZ,
  PLANE_XZ_PHIMEAN, PLANE_PHI) = range(8)
 
 
-def quantity(V, q, geo=None, cooling=None):
+def quantity(V, q, geo=None, cooling=None, spiral=None):
     """q over the whole grid, shape (NX, NY, NZ).  geo = the oracle's
-    geometry tables [6][NX][NZ] (DIVB); cooling(rho, pr) -> Lambda (ONEOTCOOL)."""
+    geometry tables [6][NX][NZ] (DIVB); cooling(rho, pr) -> Lambda (ONEOTCOOL);
+    spiral = spiral_tables(...) (SPIRALPOT, VPARA, VPERP)."""
     if q < 9:
         return V[q]
     rho, pr = V[0], V[7]
@@ -67,6 +93,14 @@ def quantity(V, q, geo=None, cooling=None):
         return out
     if q == Q_RHOBZ:
         return rho * b6
+    if q == Q_SPIRALPOT:
+        sp = spiral
+        return (sp["eps"] * 10.0 * sp["q"][:, None, :] * sp["szs"][None, None, :] * sp["x2s2o15"][:, None, None]
+                * sp["cosphase"][:, :, None]) * np.ones_like(rho)
+    if q in (Q_VPARA, Q_VPERP):
+        s = 1.0 / np.sqrt(1.0 + spiral["cot"] * spiral["cot"])
+        c = spiral["cot"] * s
+        return V[2] * c - V[1] * s if q == Q_VPARA else V[1] * c + V[2] * s
     raise ValueError(q)
 
 
@@ -128,7 +162,8 @@ IDL_FILES = [
 ]
 
 
-def idl_files(V, x, y, z, geo, dzm, cooling, nout, kplane=1, k100=100, jplane=1):
+def idl_files(V, x, y, z, geo, dzm, cooling, nout, kplane=1, k100=100, jplane=1, spiral=None,
+              azim_i=(0, 0, 0, 0), azim_k=0):
     """{relative path: bytes} of the files cylmhd_idl_output writes."""
     NX, NY, NZ = V.shape[1:]
     k100 = min(k100, NZ - 2)
@@ -147,6 +182,16 @@ def idl_files(V, x, y, z, geo, dzm, cooling, nout, kplane=1, k100=100, jplane=1)
             cache[q] = quantity(V, q, geo, cooling)
         vals = plane(cache[q], kind, idx[which], 0, dzm)
         out[os.path.join(d, "%s-%d.dat" % (name, nout))] = text(vals)
+    # azimuthal profiles at (azim_i[K], azim_k): phiout0..3.pro, fourieramp.pro
+    ak = azim_k or 1
+    for K in range(4):
+        ai = azim_i[K] or 1 + (K + 1) * (NX - 2) // 5
+        ad = "azimuth-%d" % K
+        out[os.path.join(ad, "phi.dat")] = text(y[1:-1])
+        for name, q in (("potential", Q_SPIRALPOT), ("rho", 0), ("te", Q_TEMP), ("vperp", Q_VPERP),
+                        ("vpara", Q_VPARA), ("velocitypara", Q_VPARA)):
+            Q = quantity(V, q, geo, cooling, spiral)
+            out[os.path.join(ad, "%s-%d.dat" % (name, nout))] = text(plane(Q, PLANE_PHI, ai, ak))
     return out
 
 

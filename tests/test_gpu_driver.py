@@ -41,12 +41,17 @@ def test_driver_text_output_and_converter(gpu, tmp_path):
     gdl_run = read_dir(out)
     assert sum(f.startswith("2d-density-rphi-") for f in gdl_run) == len(dumps)
     idl_run = {d: read_dir(tmp_path / "idl_run" / d) for d in os.listdir(tmp_path / "idl_run")}
-    assert sorted(idl_run) == sorted("idl%d" % n for n in dumps)
-    assert all(len(v) == 25 for v in idl_run.values())     # 4 coordinate lists + 21 slices
+    assert sorted(d for d in idl_run if d.startswith("idl")) == sorted("idl%d" % n for n in dumps)
+    assert sorted(d for d in idl_run if d.startswith("azimuth")) == ["azimuth-%d" % K for K in range(4)]
+    assert all(len(v) == 25 for d, v in idl_run.items() if d.startswith("idl"))   # 4 coordinate lists + 21 slices
+    assert all(len(v) == 1 + 6 * len(dumps) for d, v in idl_run.items() if d.startswith("azimuth"))
     run(common + ["--convert", "0", str(dumps[-1]), "--gdl", "--idl", str(tmp_path / "idl_conv")])
     assert read_dir(out) == gdl_run
     idl_conv = {d: read_dir(tmp_path / "idl_conv" / d) for d in os.listdir(tmp_path / "idl_conv")}
-    assert idl_conv == idl_run
+    # the same files, except the arm potential: the converter evaluates it at
+    # nout * tint (the restart's time, main.c:269), the run at the dump time
+    strip = lambda dd: {d: {f: b for f, b in v.items() if not f.startswith("potential")} for d, v in dd.items()}
+    assert strip(idl_conv) == strip(idl_run)
 
 
 def test_driver_converter_rejects_other_grid(gpu, tmp_path):

@@ -29,15 +29,27 @@ PLANES = [(cm.PLANE_RPHI, 1, 0), (cm.PLANE_RPHI, 4, 0), (cm.PLANE_RPHI, 0, 0), (
 
 
 def stepped(path, nsteps=2):
-    """A 20 x 16 x 12 disc with power-law cooling on, nsteps steps on the path."""
+    """A 20 x 16 x 12 disc with power-law cooling and 5 % spiral arms on,
+    nsteps steps on the path."""
     p, Ui, Vi, Uc, Vc = disc_case(1, 20, 16, 12, cooling=cm.COOL_POWERLAW, cool_lambda0=0.05, cool_tmin=0.0,
-                                  cool_index=0.5)
+                                  cool_index=0.5, epssp0=0.05)
     p.path = path
     g = cm.Context(p)
     g.set_initial(Ui, Vi)
     g.upload(Uc, Vc)
-    g.run(nsteps)
+    t, dts = g.run(nsteps)
+    g.t_gravity = t - dts[-1] if dts else 0.0     # time of the last SpiralGravity
     return p, g
+
+
+def spiral(p, g):
+    x, y, z = cm.grid(p)
+    return osl.spiral_tables(p, x, y, z, g.t_gravity)
+
+
+# quantities whose device and host values can differ in the last place
+# (libm's cos / tan of the arm phase and pitch): compared to 1e-12 relative
+NEAR = (cm.Q_SPIRALPOT, cm.Q_VPARA, cm.Q_VPERP)
 
 
 @pytest.mark.parametrize("path", PATHS, ids=[cm.PATH_NAMES[q] for q in PATHS])
@@ -47,14 +59,19 @@ def test_slices_match_restatement(gpu, path):
     geo, dzm = o.geometry(), o.dzm()
     cool = lambda rho, pr: o.L.orc_cooling(p, rho, pr)
     _, V = g.download()
+    sp = spiral(p, g)
     for q in range(cm.NQ):
-        Q = osl.quantity(V, q, geo, cool)
+        Q = osl.quantity(V, q, geo, cool, sp)
         for kind, a, b in PLANES:
             got = g.slice(q, kind, a, b)
             want = osl.plane(Q, kind, a, b, dzm)
             print(f"{cm.PATH_NAMES[path]} q={q} plane={kind} a={a} b={b} n={got.size}", flush=True)
             if q == cm.Q_ONEOTCOOL:
                 np.testing.assert_allclose(got, want, rtol=1e-14, atol=0)
+            elif q in NEAR:
+                # relative to the quantity's magnitude (the phi-mean of the
+                # arm potential is a cancellation: roundoff of that size)
+                np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * max(np.abs(Q).max(), 1e-300))
             else:
                 np.testing.assert_array_equal(got, want)
     oc = osl.plane(osl.quantity(V, cm.Q_ONEOTCOOL, geo, cool), cm.PLANE_RPHI, 1)
@@ -108,12 +125,18 @@ def test_idl_output_matches_restatement(gpu, path, tmp_path):
     cool = lambda rho, pr: o.L.orc_cooling(p, rho, pr)
     _, V = g.download()
     x, y, z = cm.grid(p)
+    sp = spiral(p, g)
+    assert sp["eps"] > 0
     for nout in (0, 12):
-        g.idl_output(str(tmp_path), nout, kplane=2, k100=100, jplane=3)
-        want = osl.idl_files(V, x, y, z, o.geometry(), o.dzm(), cool, nout, kplane=2, k100=100, jplane=3)
+        g.idl_output(str(tmp_path), nout, kplane=2, k100=100, jplane=3, azim_i=(3, 0, 9, 0), azim_k=2)
+        want = osl.idl_files(V, x, y, z, o.geometry(), o.dzm(), cool, nout, kplane=2, k100=100, jplane=3,
+                             spiral=sp, azim_i=(3, 0, 9, 0), azim_k=2)
         for rel, b in want.items():
             got = open(os.path.join(tmp_path, rel), "rb").read()
-            if "oneocoolingtime" in rel:
+            if any(s in rel for s in ("potential", "vpara", "vperp", "velocitypara")):
+                gv, wv = np.array(got.split(), float), np.array(b.split(), float)
+                np.testing.assert_allclose(gv, wv, rtol=0, atol=2e-6 * max(np.abs(wv).max(), 1e-300))
+            elif "oneocoolingtime" in rel:
                 np.testing.assert_allclose(np.array(got.split(), float), np.array(b.split(), float), rtol=1e-5)
             else:
                 assert got == b, rel

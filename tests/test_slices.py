@@ -193,6 +193,45 @@ def test_derived_quantities():
                                rtol=1e-14)
 
 
+def test_spiral_potential_matches_the_arm_force():
+    """-(1/r) dPhi/dphi of the arm potential against the phi component of
+    the oracle's spiral gravity (the reference's SpiralGravity, pinned by the
+    compiled reference): central differences, error O(dphi^2)."""
+    p = disc_params(2, 24, 64, 10)
+    t = 0.5 * p.tausp                               # the arms at half amplitude
+    x, y, z = cm.grid(p)
+    NX, NY, NZ = p.dims
+    sp = osl.spiral_tables(p, x, y, z, t)
+    assert 0 < sp["eps"] < p.epssp0
+    phi = osl.quantity(np.zeros((9, NX, NY, NZ)), osl.Q_SPIRALPOT, spiral=sp)
+    o = Oracle(p)
+    o.gravity(t)
+    gy = o.get_gravity()[1]
+    irin = int(p.rin / p.dx0)
+    inner = (slice(irin + 1, NX - 1), slice(1, NY - 1), slice(1, NZ - 1))
+    dphi = (phi[:, 2:, :] - phi[:, :-2, :]) / (y[2:] - y[:-2])[None, :, None]
+    f = -dphi / x[:, None, None]
+    want = gy[inner]
+    got = f[irin + 1:NX - 1, :, 1:NZ - 1]
+    assert np.abs(want).max() > 0
+    err = np.abs(got - want).max() / np.abs(want).max()
+    dm = p.armnum * (y[2] - y[1])
+    assert err <= 0.5 * dm * dm, err                # (m dphi)^2 / 6 of a pure cosine, with margin
+
+
+def test_arm_frame_velocities():
+    p = disc_params(2, 16, 16, 10)
+    NX, NY, NZ = p.dims
+    x, y, z = cm.grid(p)
+    sp = osl.spiral_tables(p, x, y, z, 0.0)
+    rng = np.random.default_rng(5)
+    V = rng.uniform(-1.0, 1.0, (9, NX, NY, NZ))
+    va, vp = osl.quantity(V, osl.Q_VPARA, spiral=sp), osl.quantity(V, osl.Q_VPERP, spiral=sp)
+    np.testing.assert_allclose(va * va + vp * vp, V[1] ** 2 + V[2] ** 2, rtol=1e-13)
+    V[1] = 0.0                                      # pure rotation: vpara = vphi cos(pitch)
+    np.testing.assert_allclose(osl.quantity(V, osl.Q_VPARA, spiral=sp), V[2] * np.cos(p.pitch), rtol=1e-13)
+
+
 def test_idl_file_set_matches_scripts():
     """Every file name the restatement writes is one an IDL script opens."""
     ref = "/root/reference"
@@ -207,6 +246,12 @@ def test_idl_file_set_matches_scripts():
     names = {n for n, *_ in osl.IDL_FILES}
     assert names <= opened, names - opened
     missing = opened - names
-    # not written: the azimuthal profiles of phiout*.pro / fourieramp.pro live
-    # under ./azimuth-K/, not ./idlN/
     assert not missing, missing
+    # the azimuthal profiles of phiout*.pro / fourieramp.pro under ./azimuth-K/
+    azim = set()
+    for f in os.listdir(ref):
+        if f.endswith(".pro"):
+            txt = open(os.path.join(ref, f), errors="replace").read()
+            azim |= set(re.findall(r"'\./azimuth-\d/([a-z]+)-'\+idlnumstring", txt))
+    written = {"potential", "rho", "te", "vperp", "vpara", "velocitypara"}
+    assert azim == written, azim ^ written
