@@ -353,6 +353,9 @@ int cylmhd_set_initial(cylmhd_ctx *c, const double *U, const double *V)
     if (!c || !U || !V) return CYLMHD_EINVAL;
     const int NX = c->g.NX, NY = c->g.NY, NZ = c->g.NZ;
     const size_t N = c->ds.N, NP = (size_t)NX * NY;
+    /* the copies below go through the null stream: first let the work
+     * already queued on the context stream (a step reading Uinit) finish */
+    HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(c->ds.Uinit, U, 9 * N * sizeof(double), hipMemcpyHostToDevice));
     std::vector<double> vz(9 * 2 * NP);
     for (int m = 0; m < 9; m++)
@@ -371,7 +374,7 @@ int cylmhd_set_initial(cylmhd_ctx *c, const double *U, const double *V)
             }
     HIPCHK(hipMemcpy(c->ds.Vinit_y, vy.data(), vy.size() * sizeof(double), hipMemcpyHostToDevice));
     if (c->p.path != CYLMHD_PATH_PLAIN64) {
-        int rc = fused_set_rhofloor(c->stream, c->fs, c->ds, c->p.rhocr);
+        int rc = fused_set_initial_tables(c->stream, c->fs, c->dg, c->ds, c->p.rhocr);
         if (rc) return rc;
     }
     return cylmhd_upload_state(c, U, V);

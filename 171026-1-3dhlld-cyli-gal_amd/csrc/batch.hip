@@ -99,10 +99,11 @@ __global__ __launch_bounds__(256) void k_setdt_batch(const FArgs<float> *__restr
     for (size_t q = (blockIdx.x - off[d]) * (size_t)256 + threadIdx.x; q < M; q += nb * 256) {
         const int i = 1 + (int)(q / (nj * nk)), j = 1 + (int)((q / nk) % nj), k = 1 + (int)(q % nk);
         const size_t cc = C3(i, j, k), cs = SI(i, j, k);
-        float u[9], v[9];
+        float u[9], v[9], ub[NOFF];
 #pragma unroll
         for (int m = 0; m < 9; m++) u[m] = a.Un[m * a.NS + cs];
-        Prec<float>::utov(a.c, u, g.oneox[i], v);
+        ub_load(a, i, k, ub);
+        Prec<float>::utov(a.c, u, ub, g.oneox[i], v);
         const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
         const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
         const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
@@ -315,6 +316,7 @@ int build(cylmhd_batch *b, bool keep_time)
             a.N = v.ds.N;
             a.NS = v.fs->NS; a.NZP = v.fs->NZP;
             a.rhofloor = (const float *)v.fs->rhofloor;
+            a.ubar = (const float *)v.fs->ubar;
             a.Uinit = v.ds.Uinit;
             a.Vinit_b = b->lim == CYLMHD_LIMITER_VANLEER ? v.ds.Vinit_y : v.ds.Vinit_z;
             a.edge = (const float *)v.fs->edge;

@@ -21,6 +21,9 @@ struct FusedState {
      *           reads them from consecutive addresses) */
     void *Un = nullptr, *A = nullptr, *B = nullptr, *shadow = nullptr;
     void *rhofloor = nullptr;  /* [N] rhocr * Uinitial[0] */
+    /* FP32 state only: the stored values of components 0, 2, 5, 7 are
+     * deviations from ubar [NX*NZ][4], the phi-mean of the initial state */
+    void *ubar = nullptr;
     void *edge = nullptr;      /* [2][9][NY][NZ] x-boundary face fluxes */
     double *u64_scratch = nullptr;   /* [9][N] FP64 export buffer */
     double *dtpart = nullptr;  /* Setdt partials of the last stage-3 launch */
@@ -53,7 +56,7 @@ using AllocFn = std::function<int(size_t, void **)>;
 /* one set of entry points per state precision (fused.hip built twice) */
 #define CYL_FUSED_DECLS                                                                                   \
     int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc);                       \
-    int set_rhofloor(hipStream_t s, FusedState &fs, const DevState &d, double rhocr);                     \
+    int set_initial_tables(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, double rhocr); \
     int load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const double *U64); \
     int export_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,  \
                      double *V64);                                                                        \
@@ -69,10 +72,11 @@ inline int fused_alloc(FusedState &fs, const DevGeom &g, const DevState &d, bool
 {
     return fp32 ? f32::alloc(fs, g, d, a) : f64::alloc(fs, g, d, a);
 }
-/* rhofloor = rhocr * Uinitial[0] (saveinitial.h:44), from d.Uinit */
-inline int fused_set_rhofloor(hipStream_t s, FusedState &fs, const DevState &d, double rhocr)
+/* from d.Uinit: rhofloor = rhocr * Uinitial[0] (saveinitial.h:44) and,
+ * for FP32 state, the offsets ubar */
+inline int fused_set_initial_tables(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, double rhocr)
 {
-    return fs.fp32 ? f32::set_rhofloor(s, fs, d, rhocr) : f64::set_rhofloor(s, fs, d, rhocr);
+    return fs.fp32 ? f32::set_initial_tables(s, fs, g, d, rhocr) : f64::set_initial_tables(s, fs, g, d, rhocr);
 }
 inline int fused_load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const double *U64)
 {

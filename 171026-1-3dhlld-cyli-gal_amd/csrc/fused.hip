@@ -68,10 +68,11 @@ __global__ __launch_bounds__(256) void k_setdt_state(FArgs<T> a)
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < M; q += (size_t)gridDim.x * blockDim.x) {
         const int i = 1 + (int)(q / (nj * nk)), j = 1 + (int)((q / nk) % nj), k = 1 + (int)(q % nk);
         const size_t cc = C3(i, j, k), cs = SI(i, j, k);
-        T u[9], v[9];
+        T u[9], v[9], ub[NOFF];
 #pragma unroll
         for (int m = 0; m < 9; m++) u[m] = a.Un[m * a.NS + cs];
-        Prec<T>::utov(a.c, u, g.oneox[i], v);
+        ub_load(a, i, k, ub);
+        Prec<T>::utov(a.c, u, ub, g.oneox[i], v);
         const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
         const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
         const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
@@ -120,13 +121,44 @@ __device__ __forceinline__ size_t state_index(const DevGeom &g, int NZP, size_t 
     return m * NS + row * (size_t)NZP + k + KOFF;
 }
 
+/* offset of component m at cell r of the reference layout (FP32 state:
+ * the phi-mean of the initial state for the components with a slot; FP64:
+ * none) */
 template <typename T>
-__global__ void k_to_state(DevGeom g, int NZP, size_t NS, bool zg, const double *__restrict__ src, T *__restrict__ dst,
-                           size_t N)
+__device__ __forceinline__ double ub_of(const DevGeom &g, const T *ubar, size_t m, size_t r)
+{
+    if constexpr (sizeof(T) == 4) {
+        const int sl = oslot((int)m);
+        if (sl >= 0) {
+            const size_t NYZ = (size_t)g.NY * g.NZ, i = r / NYZ, k = r % g.NZ;
+            return (double)ubar[(i * g.NZ + k) * NOFF + sl];
+        }
+    }
+    return 0.0;
+}
+
+template <typename T>
+__global__ void k_to_state(DevGeom g, int NZP, size_t NS, bool zg, const T *__restrict__ ubar, const double *__restrict__ src,
+                           T *__restrict__ dst, size_t N)
 {
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < 9 * N; q += (size_t)gridDim.x * blockDim.x) {
         const size_t m = q / N, r = q - m * N;
-        dst[state_index(g, NZP, NS, zg, m, r)] = T(src[q]);
+        dst[state_index(g, NZP, NS, zg, m, r)] = T(src[q] - ub_of(g, ubar, m, r));
+    }
+}
+
+/* ubar[i][k][slot] = phi-mean over the interior j of Uinitial[m][i][j][k]
+ * for m = 0, 2, 5, 7, rounded to T (FP32 state only) */
+template <typename T>
+__global__ void k_ubar(DevGeom g, const double *__restrict__ Uinit, T *__restrict__ ubar, size_t N)
+{
+    const size_t nik = (size_t)g.NX * g.NZ;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < NOFF * nik; q += (size_t)gridDim.x * blockDim.x) {
+        const size_t sl = q / nik, ik = q - sl * nik, i = ik / g.NZ, k = ik - i * g.NZ;
+        const size_t m = sl == 0 ? 0 : (sl == 1 ? 2 : (sl == 2 ? 5 : 7));
+        double s = 0.0;
+        for (int j = 1; j <= g.NY - 2; j++) s += Uinit[m * N + (i * g.NY + j) * g.NZ + k];
+        ubar[ik * NOFF + sl] = T(s / (double)(g.NY - 2));
     }
 }
 
@@ -138,14 +170,17 @@ __global__ void k_rhofloor(int NZ, int NZP, const double *__restrict__ Uinit0, T
 }
 
 template <typename T>
-__global__ void k_export(DevGeom g, int NZP, size_t NS, bool zg, Consts c, const T *__restrict__ U,
-                         double *__restrict__ U64, double *__restrict__ V64, size_t N)
+__global__ void k_export(DevGeom g, int NZP, size_t NS, bool zg, Consts c, const T *__restrict__ ubar,
+                         const T *__restrict__ U, double *__restrict__ U64, double *__restrict__ V64, size_t N)
 {
     const size_t NYZ = (size_t)g.NY * g.NZ;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < N; q += (size_t)gridDim.x * blockDim.x) {
         double u[9], v[9];
 #pragma unroll
-        for (int m = 0; m < 9; m++) { u[m] = (double)U[state_index(g, NZP, NS, zg, m, q)]; U64[m * N + q] = u[m]; }
+        for (int m = 0; m < 9; m++) {
+            u[m] = (double)U[state_index(g, NZP, NS, zg, m, q)] + ub_of(g, ubar, m, q);
+            U64[m * N + q] = u[m];
+        }
         utov<double>(c, u, g.oneox[q / NYZ], v);
 #pragma unroll
         for (int m = 0; m < 9; m++) V64[m * N + q] = v[m];
@@ -153,8 +188,8 @@ __global__ void k_export(DevGeom g, int NZP, size_t NS, bool zg, Consts c, const
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag_t(DevGeom g, int NZP, bool zg, const T *__restrict__ U, size_t N,
-                                                double *__restrict__ red)
+__global__ __launch_bounds__(256) void k_diag_t(DevGeom g, int NZP, bool zg, const T *__restrict__ ubar, const T *__restrict__ U,
+                                                size_t N, double *__restrict__ red)
 {
     /* N is the component stride of the padded state arrays */
     const size_t ni = g.NX - 2, nj = g.NY - 2, nk = g.NZ - 2, M = ni * nj * nk;
@@ -164,16 +199,26 @@ __global__ __launch_bounds__(256) void k_diag_t(DevGeom g, int NZP, bool zg, con
         const int i = 1 + (int)(q / (nj * nk)), j = 1 + (int)((q / nk) % nj), k = 1 + (int)(q % nk);
         const size_t cc = (((size_t)i) * g.NY + (size_t)j) * (size_t)NZP + (size_t)k + KOFF, ik = (size_t)i * g.NZ + k;
         const double dv = g.dvol[ik];
-        const double b4 = U[4 * N + cc], b5 = U[5 * N + cc], b6 = U[6 * N + cc];
-        const double brp = 0.5 * (b4 + (double)U[4 * N + cc + si]), brm = 0.5 * (b4 + (double)U[4 * N + cc - si]);
-        const double bpp = 0.5 * (b5 + (double)U[5 * N + cc + sj]), bpm = 0.5 * (b5 + (double)U[5 * N + cc - sj]);
+        /* full values: the stored deviation + the offset of its (i, k) */
+        auto ob = [&](int ii, int kk, int m) -> double {
+            if constexpr (sizeof(T) == 4) {
+                if (oslot(m) >= 0) return (double)ubar[((size_t)ii * g.NZ + kk) * NOFF + oslot(m)];
+            }
+            return 0.0;
+        };
+        const double b4 = (double)U[4 * N + cc] + ob(i, k, 4), b5 = (double)U[5 * N + cc] + ob(i, k, 5);
+        const double b6 = (double)U[6 * N + cc] + ob(i, k, 6);
+        const double brp = 0.5 * (b4 + ((double)U[4 * N + cc + si] + ob(i + 1, k, 4)));
+        const double brm = 0.5 * (b4 + ((double)U[4 * N + cc - si] + ob(i - 1, k, 4)));
+        const double bpp = 0.5 * (b5 + ((double)U[5 * N + cc + sj] + ob(i, k, 5)));
+        const double bpm = 0.5 * (b5 + ((double)U[5 * N + cc - sj] + ob(i, k, 5)));
         /* the z neighbours k = 0 and NZ-1 are ghosts: packed planes with zg */
         const size_t zp = zg && k == g.NZ - 2 ? 9 * N + (((size_t)g.NX + i) * g.NY + j) * 9 + 6 : 6 * N + cc + 1;
         const size_t zm = zg && k == 1 ? 9 * N + ((size_t)i * g.NY + j) * 9 + 6 : 6 * N + cc - 1;
-        const double bzp = 0.5 * (b6 + (double)U[zp]), bzm = 0.5 * (b6 + (double)U[zm]);
+        const double bzp = 0.5 * (b6 + ((double)U[zp] + ob(i, k + 1, 6))), bzm = 0.5 * (b6 + ((double)U[zm] + ob(i, k - 1, 6)));
         const double db = (g.g2[ik] * brp - g.g1[ik] * brm) + (g.g3[ik] * bpp - g.g3[ik] * bpm) + (g.g5[ik] * bzp - g.g5[ik] * bzm);
-        acc[0] += (double)U[cc] * dv;
-        acc[1] += (double)U[7 * N + cc] * dv;
+        acc[0] += ((double)U[cc] + ob(i, k, 0)) * dv;
+        acc[1] += ((double)U[7 * N + cc] + ob(i, k, 7)) * dv;
         acc[2] += fabs(db) * dv;
         acc[3] += fabs((double)U[8 * N + cc]) * dv;
     }
@@ -288,6 +333,12 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     if ((rc = alloc(4 * nik * sizeof(double), &p))) return rc;
     fs.ikd = (double *)p;
     if ((rc = alloc(4 * nik * sizeof(T), &fs.ikg))) return rc;
+    if (fs.fp32) {
+        /* state offsets: zero until the initial state is known (a state
+         * uploaded before cylmhd_set_initial is stored as it is) */
+        if ((rc = alloc(NOFF * nik * sizeof(T), &fs.ubar))) return rc;
+        if (hipMemset(fs.ubar, 0, NOFF * nik * sizeof(T)) != hipSuccess) return CYLMHD_EHIP;
+    }
     hipLaunchKernelGGL(k_pack<T>, dim3(nblocks(nik, 256, 4096)), dim3(256), 0, 0, g, fs.itab, (T *)fs.ikg, fs.ikd, fs.ktab);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return CYLMHD_EHIP;
     return 0;
@@ -301,6 +352,7 @@ static FArgs<T> make_args(FusedState &fs, const DevGeom &g, const DevState &d, c
     a.N = d.N;
     a.NS = fs.NS; a.NZP = fs.NZP;
     a.rhofloor = (const T *)fs.rhofloor;
+    a.ubar = (const T *)fs.ubar;
     a.Uinit = d.Uinit; a.Vinit_b = d.Vinit_z;
     a.edge = (const T *)fs.edge;
     a.counts = d.counts;
@@ -312,9 +364,13 @@ static FArgs<T> make_args(FusedState &fs, const DevGeom &g, const DevState &d, c
     return a;
 }
 
-int set_rhofloor(hipStream_t s, FusedState &fs, const DevState &d, double rhocr)
+int set_initial_tables(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, double rhocr)
 {
     hipLaunchKernelGGL(k_rhofloor<FT>, dim3(nblocks(d.N, 256, 8192)), dim3(256), 0, s, fs.NZ, fs.NZP, d.Uinit, (FT *)fs.rhofloor, d.N, rhocr);
+    if (fs.ubar) {
+        const size_t n = NOFF * (size_t)g.NX * g.NZ;
+        hipLaunchKernelGGL(k_ubar<FT>, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, g, d.Uinit, (FT *)fs.ubar, d.N);
+    }
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
@@ -322,14 +378,14 @@ int load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &
 {
     (void)g;
     const size_t n = 9 * d.N;
-    hipLaunchKernelGGL(k_to_state<FT>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, U64, (FT *)fs.Un, d.N);
+    hipLaunchKernelGGL(k_to_state<FT>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, (const FT *)fs.ubar, U64, (FT *)fs.Un, d.N);
     fs.dt_valid = false;
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
 int export_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c, double *V64)
 {
-    hipLaunchKernelGGL(k_export<FT>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, c, (const FT *)fs.Un, fs.u64_scratch, V64, d.N);
+    hipLaunchKernelGGL(k_export<FT>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, c, (const FT *)fs.ubar, (const FT *)fs.Un, fs.u64_scratch, V64, d.N);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
@@ -452,7 +508,7 @@ int diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d)
 {
     const size_t M = (size_t)(g.NX - 2) * (g.NY - 2) * (g.NZ - 2);
     const int nb = (int)nblocks(M, 256, (unsigned)(d.nred / 4));
-    hipLaunchKernelGGL(k_diag_t<FT>, dim3(nb), dim3(256), 0, s, g, fs.NZP, fs.zg, (const FT *)fs.Un, fs.NS, d.red);
+    hipLaunchKernelGGL(k_diag_t<FT>, dim3(nb), dim3(256), 0, s, g, fs.NZP, fs.zg, (const FT *)fs.ubar, (const FT *)fs.Un, fs.NS, d.red);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, d.red, nb);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }

@@ -81,15 +81,28 @@ __device__ __forceinline__ X ro(const X *p, size_t idx)
 }
 
 
-/* ---- precision-specific UtoV and Check ---------------------------------- */
+/* ---- precision-specific UtoV and Check ----------------------------------
+ * The FP32 state is stored as its deviation from a fixed per-(i,k) offset,
+ * u = U - ubar(i,k), ubar the phi-mean of the initial state for the four
+ * components that carry a disc's axisymmetric background -- rho, r rho
+ * vphi, B_phi and E (NOFF below); the others are stored as they are, being
+ * small themselves: the RK combinations are affine with weights summing
+ * to one, so they act on the deviations unchanged, and the rounding of each
+ * stored value is relative to the deviation, not to U.  In a high-Mach
+ * low-beta cell E is hundreds of times p/(gamma-1): rounding E itself to
+ * FP32 every stage would put ~1e-5 relative noise into p, which the flow
+ * near the inner edge amplifies.  The full values are formed in FP64 where
+ * they feed the pressure recovery.  FP64 state keeps no offset (ub unused). */
+constexpr int NOFF = 4;                  /* offset slots: components 0, 2, 5, 7 */
+__host__ __device__ constexpr int oslot(int m) { return m == 0 ? 0 : (m == 2 ? 1 : (m == 5 ? 2 : (m == 7 ? 3 : -1))); }
 template <typename T> struct Prec;
 
 template <> struct Prec<double> {
-    static __device__ __forceinline__ void utov(const Consts &c, const double *U, double oneox, double *V)
+    static __device__ __forceinline__ void utov(const Consts &c, const double *U, const double *, double oneox, double *V)
     {
         cyl::utov<double>(c, U, oneox, V);
     }
-    static __device__ __forceinline__ int check(const Consts &c, double *U, double *V, double x, double rf)
+    static __device__ __forceinline__ int check(const Consts &c, double *U, const double *, double *V, double x, double rf)
     {
         return cyl::check<double>(c, U, V, x, rf);
     }
@@ -97,25 +110,31 @@ template <> struct Prec<double> {
 
 /* FP32 state: p = gm1 (E - (rho v)^2/2rho - B^2/2) and the energy rebuild
  * of Check are formed in FP64 (the cancellation points of low-beta and
- * high-Mach cells); the rest stays FP32. */
+ * high-Mach cells); the rest stays FP32.  U holds deviations, ub the
+ * offsets of the cell. */
 template <> struct Prec<float> {
-    static __device__ __forceinline__ void utov(const Consts &c, const float *U, double oneox, float *V)
+    static __device__ __forceinline__ void utov(const Consts &c, const float *U, const float *ub, double oneox, float *V)
     {
-        const float oneorho = 1.0f / U[0];
-        const double d1 = U[1], d2 = U[2], d3 = U[3], d4 = U[4], d5 = U[5], d6 = U[6];
+        /* full values of the offset components, formed in FP64 */
+        const double d0 = (double)U[0] + (double)ub[0];
+        const double d2 = (double)U[2] + (double)ub[1];
+        const double d5 = (double)U[5] + (double)ub[2];
+        const float rho = (float)d0, mphi = (float)d2;
+        const float oneorho = 1.0f / rho;
+        const double d1 = U[1], d3 = U[3], d4 = U[4], d6 = U[6];
         const double b2 = d4 * d4 + d5 * d5 + d6 * d6;
         const double r2v2 = d1 * d1 + d2 * d2 * oneox * oneox + d3 * d3;
-        V[0] = U[0];
+        V[0] = rho;
         V[1] = U[1] * oneorho;
-        V[2] = U[2] * oneorho * (float)oneox;
+        V[2] = mphi * oneorho * (float)oneox;
         V[3] = U[3] * oneorho;
         V[4] = U[4];
-        V[5] = U[5];
+        V[5] = (float)d5;
         V[6] = U[6];
-        V[7] = (float)(c.gm1 * ((double)U[7] - 0.5 * (r2v2 / (double)U[0] + b2)));
+        V[7] = (float)(c.gm1 * (((double)U[7] + (double)ub[3]) - 0.5 * (r2v2 / d0 + b2)));
         V[8] = U[8];
     }
-    static __device__ __forceinline__ int check(const Consts &c, float *U, float *V, double x, float rf)
+    static __device__ __forceinline__ int check(const Consts &c, float *U, const float *ub, float *V, double x, float rf)
     {
         int flags = 0;
         const double v2 = (double)V[2] * V[2];
@@ -123,23 +142,23 @@ template <> struct Prec<float> {
         if (V[0] < rf) {
             flags |= 1;
             V[0] = rf; V[1] = 0.0f; V[3] = 0.0f;
-            U[0] = V[0]; U[1] = 0.0f; U[2] = (float)(x * V[0] * V[2]); U[3] = 0.0f;
-            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7]);
+            U[0] = (float)((double)V[0] - ub[0]); U[1] = 0.0f; U[2] = (float)(x * V[0] * V[2] - ub[1]); U[3] = 0.0f;
+            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7] - ub[3]);
         }
         const float prfloor = V[0] * (float)c.prcr;
         if (V[7] < prfloor) {
             flags |= 2;
             V[1] = 0.0f; V[3] = 0.0f; V[7] = prfloor;
             U[1] = 0.0f; U[3] = 0.0f;
-            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7]);
+            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7] - ub[3]);
         }
         const double oneobeta = 0.5 * b2 / (double)V[7];
         if (oneobeta > c.oneobetacr) {
             flags |= 4;
             V[0] = (float)(c.betacr * oneobeta * V[0]); V[1] = 0.0f; V[3] = 0.0f;
             V[7] = (float)(c.betacr * oneobeta * V[7]);
-            U[0] = V[0]; U[1] = 0.0f; U[2] = (float)(x * V[0] * V[2]); U[3] = 0.0f;
-            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7]);
+            U[0] = (float)((double)V[0] - ub[0]); U[1] = 0.0f; U[2] = (float)(x * V[0] * V[2] - ub[1]); U[3] = 0.0f;
+            U[7] = (float)(0.5 * ((double)V[0] * v2 + b2) + c.oneogm1 * V[7] - ub[3]);
         }
         return flags;
     }
@@ -157,6 +176,8 @@ template <typename T> struct FArgs {
     const T *Uin, *Un;
     T *Uout, *shadow;
     const T *rhofloor;
+    /* FP32 state offsets ubar [NX*NZ][NOFF] (FusedState::ubar); unused for FP64 */
+    const T *ubar;
     const double *Uinit;
     /* initial-state planes an interface boundary reads: Vinit_z [9][2][NX][NY]
      * (MC, ZVlrFixed) or Vinit_y [9][2][NX][NZ] (vanLeer, YVlrInitialize) */
@@ -179,32 +200,40 @@ __device__ __forceinline__ int jmap(const DevGeom &g, int jg)
 }
 __device__ __forceinline__ int kmap(const DevGeom &g, int kg) { return kg < 0 ? 0 : (kg > g.NZ - 1 ? g.NZ - 1 : kg); }
 
-/* stage input of one cell as primitives: psi damped (GLMSourceSolver) then UtoV */
+/* offsets of cell (i, k) (FP32 state; the FP64 build keeps none) */
 template <typename T>
-__device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, T *v, size_t stride)
+__device__ __forceinline__ void ub_load(const FArgs<T> &a, int i, int k, T *ub)
 {
-    T u[9];
+    if constexpr (sizeof(T) == 4) {
+        const float4 q = *(const float4 *)(a.ubar + ((size_t)i * a.g.NZ + k) * NOFF);
+        ub[0] = q.x; ub[1] = q.y; ub[2] = q.z; ub[3] = q.w;
+    }
+}
+
+/* stage input of one cell (i, k) as primitives: psi damped
+ * (GLMSourceSolver) then UtoV */
+template <typename T>
+__device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, int k, T *v, size_t stride)
+{
+    T u[9], ub[NOFF];
 #pragma unroll
     for (int m = 0; m < 9; m++) u[m] = a.Uin[m * stride + cc];
+    ub_load(a, i, k, ub);
     u[8] = T(a.c.psicoef) * u[8];
-    Prec<T>::utov(a.c, u, ro(a.itab, (size_t)i * IT + IT_OX), v);
+    Prec<T>::utov(a.c, u, ub, ro(a.itab, (size_t)i * IT + IT_OX), v);
 }
 template <typename T>
-__device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, T *v)
+__device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, int k, T *v)
 {
-    T u[9];
-#pragma unroll
-    for (int m = 0; m < 9; m++) u[m] = a.Uin[m * a.NS + cc];
-    u[8] = T(a.c.psicoef) * u[8];
-    Prec<T>::utov(a.c, u, ro(a.itab, (size_t)i * IT + IT_OX), v);
+    cell_V(a, cc, i, k, v, a.NS);
 }
 /* cell (i, j, k) of any k; ZGL: the batch layout's z-side ghosts */
 template <bool ZGL, typename T>
 __device__ __forceinline__ void cell_V_at(const FArgs<T> &a, int i, int j, int k, T *v)
 {
     const DevGeom &g = a.g;
-    if (ZGL && (k == 0 || k == g.NZ - 1)) cell_V(a, ZG(k != 0, i, j), i, v, 1);
-    else cell_V(a, SI(i, j, k), i, v);
+    if (ZGL && (k == 0 || k == g.NZ - 1)) cell_V(a, ZG(k != 0, i, j), i, k, v, 1);
+    else cell_V(a, SI(i, j, k), i, k, v);
 }
 
 /* Work-group barrier for LDS traffic only.  __syncthreads() fences global
@@ -401,7 +430,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         if (is_cell && p >= 0 && p <= g.NX - 1) {
             T v[9];
             if constexpr (BATCH) cell_V_at<true>(a, p, jmap(g, gj), kmap(g, gk), v);
-            else cell_V(a, SI(p, jmap(g, gj), kmap(g, gk)), p, v);
+            else cell_V(a, SI(p, jmap(g, gj), kmap(g, gk)), p, kmap(g, gk), v);
             const int off = (p % 3) * SLOT + cbase;
 #pragma unroll
             for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
@@ -418,7 +447,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         if (t < NHALO) {
             T v[9];
             if constexpr (BATCH) cell_V_at<true>(a, p, jmap(g, j0 + hj), kmap(g, k0 + hk), v);
-            else cell_V(a, (size_t)p * g.NY * a.NZP + hcol, p, v);
+            else cell_V(a, (size_t)p * g.NY * a.NZP + hcol, p, kmap(g, k0 + hk), v);
             const int off = (p % 3) * SLOT + hoff;
 #pragma unroll
             for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
@@ -494,10 +523,11 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
          * the state pass; this pays where the extra live registers do not
          * spill (the batch kernel), and not in the single-disc kernel */
         const bool pre = i + 2 <= g.NX - 1 && i < g.NX - 2;
-        T raw[9];
+        T raw[9], rub[NOFF];
         if constexpr (BATCH) {
             if (pre && is_cell) {
                 const int kk = kmap(g, gk);
+                ub_load(a, i + 2, kk, rub);
                 const bool zg = kk == g.NZ - 1;        /* past the end of a partial tile */
                 const size_t cp = zg ? ZG(1, i + 2, jmap(g, gj)) : SI(i + 2, jmap(g, gj), kk);
                 const size_t st = zg ? (size_t)1 : NS;
@@ -539,7 +569,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             if (pre && is_cell) {
                 T v[9];
                 raw[8] = T(c.psicoef) * raw[8];
-                Prec<T>::utov(c, raw, ro(a.itab, (size_t)(i + 2) * IT + IT_OX), v);
+                Prec<T>::utov(c, raw, rub, ro(a.itab, (size_t)(i + 2) * IT + IT_OX), v);
                 const int off = ((i + 2) % 3) * SLOT + cbase;
 #pragma unroll
                 for (int m = 0; m < 9; m++) ring[off + m * PS] = v[m];
@@ -586,9 +616,9 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     /* ZVlrPeriodicBoundary: the upper state of cell NZ-2 */
                     T vm[9], v0[9], vp[9];
                     const int jj = jmap(g, j0 + fzj), L = g.NZ - 2;
-                    cell_V(a, SI(i, jj, L - 1), i, vm); cell_V(a, SI(i, jj, L), i, v0);
+                    cell_V(a, SI(i, jj, L - 1), i, L - 1, vm); cell_V(a, SI(i, jj, L), i, L, v0);
                     if constexpr (BATCH) cell_V_at<true>(a, i, jj, L + 1, vp);
-                    else cell_V(a, SI(i, jj, L + 1), i, vp);
+                    else cell_V(a, SI(i, jj, L + 1), i, L + 1, vp);
                     cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), vl, (T *)nullptr);
                 } else {
                     /* ZLeftVlVzAntisymBzsym: lower state of cell 2, vz negated */
@@ -612,8 +642,8 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     T vm[9], v0[9], vp[9];
                     const int jj = jmap(g, j0 + fzj);
                     if constexpr (BATCH) cell_V_at<true>(a, i, jj, 0, vm);
-                    else cell_V(a, SI(i, jj, 0), i, vm);
-                    cell_V(a, SI(i, jj, 1), i, v0); cell_V(a, SI(i, jj, 2), i, vp);
+                    else cell_V(a, SI(i, jj, 0), i, 0, vm);
+                    cell_V(a, SI(i, jj, 1), i, 1, v0); cell_V(a, SI(i, jj, 2), i, 2, vp);
                     cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), (T *)nullptr, vr);
                 } else {
                     /* ZRightVrFreeBoundary (minmod) / ZVlrFreeBoundary (vanLeer): Vr = Vl */
@@ -687,6 +717,8 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             const bool spiral = gravity_staged(g, i, it, s_ikd[sl][ck], s_kt[ck], s_sc[sl][cj], gx, gy, gz,
                                                ALT && (c.source == 1 || c.source == 2));
             sources<T, ALT>(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
+            T ubc[NOFF];                 /* offsets of the cell (after the RK combination) */
+            ub_load(a, i, gk, ubc);
 #pragma unroll
             for (int m = 0; m < 9; m++) {
                 /* the empty asm keeps the (zero or loaded) Un value in its
@@ -747,6 +779,11 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     double ui[9];                 /* every load issued before the first use */
 #pragma unroll
                     for (int m = 0; m < 9; m++) ui[m] = a.Uinit[m * N + ccr];
+                    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+                        for (int m = 0; m < 9; m++)
+                            if (oslot(m) >= 0) ui[m] -= (double)ubc[oslot(m)];   /* the target as a deviation */
+                    }
 #pragma unroll
                     for (int m = 0; m < 9; m++) {
                         if (ax) u[m] -= (T(it[IT_XABS]) * (u[m] - T(ui[m])));
@@ -755,8 +792,8 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
                     }
                 }
             }
-            Prec<T>::utov(c, u, it[IT_OX], v);
-            const int fl = Prec<T>::check(c, u, v, it[IT_X], rfl);
+            Prec<T>::utov(c, u, ubc, it[IT_OX], v);
+            const int fl = Prec<T>::check(c, u, ubc, v, it[IT_X], rfl);
             f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
 #pragma unroll
             for (int m = 0; m < 9; m++) a.Uout[m * NS + cc] = u[m];
@@ -819,9 +856,9 @@ __device__ __forceinline__ void edge_cell(const FArgs<T> &a, T *edge, int q)
     const int j = 1 + q / nk, k = 1 + q % nk;
     T vm[9], v0[9], vp[9], up[9], lo[9], f[9];
     const int L = g.NX - 2;
-    cell_V(a, SI(L - 1, j, k), L - 1, vm); cell_V(a, SI(L, j, k), L, v0); cell_V(a, SI(L + 1, j, k), L + 1, vp);
+    cell_V(a, SI(L - 1, j, k), L - 1, k, vm); cell_V(a, SI(L, j, k), L, k, v0); cell_V(a, SI(L + 1, j, k), L + 1, k, vp);
     cell_states<0, LIM>(a.c, vm, v0, vp, T(g.oneodx[L]), T(g.oneodx[L - 1]), T(1.0), up, (T *)nullptr);
-    cell_V(a, SI(0, j, k), 0, vm); cell_V(a, SI(1, j, k), 1, v0); cell_V(a, SI(2, j, k), 2, vp);
+    cell_V(a, SI(0, j, k), 0, k, vm); cell_V(a, SI(1, j, k), 1, k, v0); cell_V(a, SI(2, j, k), 2, k, vp);
     cell_states<0, LIM>(a.c, vm, v0, vp, T(g.oneodx[1]), T(g.oneodx[0]), T(1.0), (T *)nullptr, lo);
     const size_t NE = (size_t)g.NY * g.NZ, e = (size_t)j * g.NZ + k;
     riemann<0, HLL>(a.c, LIM == 2 ? lo : up, lo, T(g.xm[0]), T(g.oneox[0]), f);
@@ -868,27 +905,51 @@ __device__ __forceinline__ void halo_range(const FArgs<T> &a, size_t first, size
         const int zsl = sk == 1 ? 0 : (sk == 2 ? 1 : 2);
         const T *sp = zside ? a.shadow + 9 * NS + 9 * ((size_t)zsl * P + (size_t)si * NY + sj) : a.shadow + SI(si, sj, sk);
         const size_t sstride = zside ? 1 : NS;
-        T u[9], v[9];
+        T u[9], v[9], ub[NOFF];
+        ub_load(a, i, k, ub);
+        if constexpr (sizeof(T) == 4) {
+            /* the source's deviation -> its full value (offsets of (si, sk)),
+             * signed, -> a deviation from the ghost's own offsets; exact
+             * where the two offsets agree and no sign changes */
+            T us[NOFF];
+            ub_load(a, si, sk, us);
 #pragma unroll
-        for (int m = 0; m < 9; m++) {
-            const T val = sp[m * sstride];
-            u[m] = (sign >> m) & 1u ? -val : val;
+            for (int m = 0; m < 9; m++) {
+                const T val = sp[m * sstride];
+                if (oslot(m) >= 0) {
+                    const double full = (double)val + (double)us[oslot(m)];
+                    u[m] = (float)(((sign >> m) & 1u ? -full : full) - (double)ub[oslot(m)]);
+                } else {
+                    u[m] = (sign >> m) & 1u ? -val : val;
+                }
+            }
+            if (zero3) u[3] = T(0.0);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 9; m++) {
+                const T val = sp[m * sstride];
+                u[m] = (sign >> m) & 1u ? -val : val;
+            }
+            if (zero3) u[3] = T(0.0);
         }
-        if (zero3) u[3] = T(0.0);
         if (a.last) {
             const bool ax = i >= g.NX - 11, ac = i <= g.irin, az = !a.rk3 && k >= g.NZ - 11;
             if (ax || ac || az) {
 #pragma unroll
                 for (int m = 0; m < 9; m++) {
-                    const T ui = T(a.Uinit[m * N + dstr]);
+                    double uid = a.Uinit[m * N + dstr];
+                    if constexpr (sizeof(T) == 4) {
+                        if (oslot(m) >= 0) uid -= (double)ub[oslot(m) >= 0 ? oslot(m) : 0];
+                    }
+                    const T ui = T(uid);
                     if (ax) u[m] -= (T(g.xabs[i - (g.NX - 11)]) * (u[m] - ui));
                     if (ac) u[m] -= (T(g.cabs[(size_t)i * g.NZ + k]) * (u[m] - ui));
                     if (az) u[m] -= (T(g.zabs[k - (g.NZ - 11)]) * (u[m] - ui));
                 }
             }
         }
-        Prec<T>::utov(a.c, u, g.oneox[i], v);
-        const int fl = Prec<T>::check(a.c, u, v, g.x[i], a.rhofloor[dst]);
+        Prec<T>::utov(a.c, u, ub, g.oneox[i], v);
+        const int fl = Prec<T>::check(a.c, u, ub, v, g.x[i], a.rhofloor[dst]);
         f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
         /* A z-side ghost is the only cell of its 64-byte line (the rest is
          * row padding: KOFF cells before k = 1, NZP - NZ - KOFF after NZ-1),

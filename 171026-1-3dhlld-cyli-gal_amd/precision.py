@@ -7,7 +7,8 @@ Reported per conserved variable: relative L1 and L-infinity error over the
 interior, and over the cells outside the absorbing layers (the evolved
 solution), with the components that vanish in the initial disc (rho vr,
 rho vz, Br, Bz, psi) measured against the vector they belong to (rho v,
-B); plus L1 div B of both runs and the floor hits of the FP32 run.
+B); the plain per-component L1 (every component against its own sum);
+plus L1 div B of both runs and the floor hits of the FP32 run.
 """
 import numpy as np
 
@@ -83,6 +84,8 @@ def compare(p, steps, ref_path=cm.PATH_PLAIN64, test_path=cm.PATH_FUSED32, log=N
         d32 = g.diagnostics()
         U32, _ = g.download()
     l1, linf = errors(U32, U64, x)
+    sl = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    l1_plain = np.abs(U32[sl] - U64[sl]).sum(axis=(1, 2, 3)) / np.maximum(np.abs(U64[sl]).sum(axis=(1, 2, 3)), 1e-300)
     ilo, ihi = physical_range(p)
     pl1, plinf = errors(U32, U64, x, ilo, ihi)
     del U32, U64
@@ -92,6 +95,7 @@ def compare(p, steps, ref_path=cm.PATH_PLAIN64, test_path=cm.PATH_FUSED32, log=N
         l1={k: float(v) for k, v in zip(NAMES, l1)},
         linf={k: float(v) for k, v in zip(NAMES, linf)},
         l1_max=float(l1.max()),
+        l1_plain={k: float(v) for k, v in zip(NAMES, l1_plain)},
         physical={"i_range": [ilo, ihi],
                   "l1": {k: float(v) for k, v in zip(NAMES, pl1)},
                   "linf": {k: float(v) for k, v in zip(NAMES, plinf)},

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import cylmhd as cm
-from cases import l1_rel_scaled
+from cases import group_scales, l1_rel_scaled
 
 pytestmark = pytest.mark.gpu
 
@@ -110,8 +110,12 @@ def compare_with_single(ps, nsteps=2):
             assert e.max() <= 1e-6, e
             # the z-side ghost planes, which a batch keeps packed apart, come
             # back through the download like the single disc's: the same
-            # relative L1 as the interior's (pointwise, both differ alike)
-            scale = np.abs(Us).max(axis=(1, 2, 3))[:, None, None] + 1e-30
+            # relative error as the interior's (pointwise, both differ alike),
+            # measured on the scales of the L1 bar (the components that start
+            # at zero -- rho vz, Br, Bz, psi -- against their vector: on their
+            # own noise-level magnitude the two compiled kernels' roundings
+            # differ by 1e-4 relative after two steps)
+            scale = group_scales(Us, cm.grid(p)[0], l1=False, interior=False)[:, None, None] + 1e-30
             inner = (np.abs(Ub - Us)[:, 1:-1, 1:-1, 1:-1] / scale[:, :, :, None]).max()
             for kg in (0, -1):
                 dz = np.abs(Ub[:, :, :, kg] - Us[:, :, :, kg]) / scale

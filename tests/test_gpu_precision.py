@@ -22,16 +22,17 @@ TOL_L1 = 1.0e-5
 
 
 def test_fp32_l1_error_after_100_steps(gpu):
-    """The bar is applied outside the absorbing layers (precision.physical_range):
-    inside the central one (r < 0.2) the flow next to the inner boundary is
-    driven hard, FP32 and FP64 runs part there first, and every step relaxes
-    those cells towards the initial state.  The whole-interior error is
-    reported beside it and held to 3e-5."""
+    """The bar over the whole interior, every variable: relative L1 <= 1e-5.
+    rho, r rho vphi, B_phi and E are measured against their own magnitude;
+    rho vr, rho vz, Br, Bz and psi (zero in the initial disc) against the
+    vector they belong to.  The error over the cells outside the absorbing
+    layers and the plain per-component L1 are reported beside it."""
     p = cm.params_config(3, 512, 512, 256)
     r = precision.compare(p, 100, cm.PATH_PLAIN64, cm.PATH_FUSED32, log=print)
     print(json.dumps(r))
-    assert r["physical"]["l1_max"] <= TOL_L1, r["physical"]["l1"]
-    assert r["l1_max"] <= 3.0 * TOL_L1, r["l1"]
+    assert r["l1_max"] <= TOL_L1, r["l1"]
+    for k in ("rho", "r_rho_vphi", "bphi", "e"):
+        assert r["l1_plain"][k] <= TOL_L1, r["l1_plain"]
     assert r["mass_rel"] <= 1e-7 and r["energy_rel"] <= 1e-7
     # div B of the FP32 run stays at the FP64 run's level
     assert r["divb_l1"]["test"] <= 1.5 * r["divb_l1"]["ref"] + 1e-12
