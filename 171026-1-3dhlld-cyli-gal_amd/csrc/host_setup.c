@@ -594,6 +594,55 @@ int cylmhd_vtou_host(const cylmhd_params *p, const double *V, double *U)
     return CYLMHD_OK;
 }
 
+int cylmhd_check_host(const cylmhd_params *p, const double *rho_init, double *U, double *V, int counts[3])
+{
+    /* check.h:23-188 over every cell, ghosts included; rhofloor[c] =
+     * rho_init[c] * rhocr as SaveInitial forms it (saveinitial.h:44), or 0
+     * where no SaveInitial has run yet (the reference's global array is then
+     * still zero-initialised, so the density floor cannot fire) */
+    if (!p || !U || !V) return CYLMHD_EINVAL;
+    cyl_grid g;
+    int rc = cyl_grid_build(p, &g);
+    if (rc) return rc;
+    const size_t N = (size_t)g.NX * g.NY * g.NZ;
+    const double oneobetacr = (1.0 / p->betacr), oneogm1 = p->oneogm1;
+    int rhoflag = 0, prflag = 0, oneobetaflag = 0;
+    for (int i = 0; i < g.NX; i++) {
+        const double x = g.x[i];
+        for (size_t jk = 0; jk < (size_t)g.NY * g.NZ; jk++) {
+            const size_t c = (size_t)i * g.NY * g.NZ + jk;
+            double *v0 = V + c, *v1 = V + N + c, *v3 = V + 3 * N + c, *v7 = V + 7 * N + c;
+            const double v2 = V[2 * N + c] * V[2 * N + c];
+            const double b2 = (V[4 * N + c] * V[4 * N + c] + V[5 * N + c] * V[5 * N + c] + V[6 * N + c] * V[6 * N + c]);
+            const double rf = rho_init ? rho_init[c] * p->rhocr : 0.0;
+            if (*v0 < rf) {
+                rhoflag++;
+                *v0 = rf; *v1 = 0.0; *v3 = 0.0;
+                U[c] = *v0; U[N + c] = 0.0; U[2 * N + c] = x * *v0 * V[2 * N + c]; U[3 * N + c] = 0.0;
+                U[7 * N + c] = 0.5 * (*v0 * v2 + b2) + oneogm1 * *v7;
+            }
+            const double prfloor = *v0 * p->prcr;
+            if (*v7 < prfloor) {
+                prflag++;
+                *v1 = 0.0; *v3 = 0.0; *v7 = prfloor;
+                U[N + c] = 0.0; U[3 * N + c] = 0.0;
+                U[7 * N + c] = (0.5 * (*v0 * v2 + b2) + oneogm1 * *v7);
+            }
+            const double oneobeta = 0.5 * b2 / *v7;
+            if (oneobeta > oneobetacr) {
+                oneobetaflag++;
+                *v0 = p->betacr * oneobeta * *v0; *v1 = 0.0; *v3 = 0.0;
+                *v7 = p->betacr * oneobeta * *v7;
+                U[c] = *v0; U[N + c] = 0.0; U[2 * N + c] = x * *v0 * V[2 * N + c]; U[3 * N + c] = 0.0;
+                U[7 * N + c] = (0.5 * (*v0 * v2 + b2) + oneogm1 * *v7);
+            }
+        }
+    }
+    cyl_grid_free(&g);
+    if (counts) { counts[0] = rhoflag; counts[1] = prflag; counts[2] = oneobetaflag; }
+    return CYLMHD_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 static int write_raw(const char *dir, const char *name, const void *buf, size_t bytes)
 {

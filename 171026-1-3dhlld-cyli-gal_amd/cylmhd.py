@@ -109,6 +109,7 @@ def lib():
         "cylmhd_model_disc_state": ([P, _dp, _dp], C.c_int),
         "cylmhd_fill_ghosts_host": ([P, _dp, _dp], C.c_int),
         "cylmhd_vtou_host": ([P, _dp, _dp], C.c_int),
+        "cylmhd_check_host": ([P, C.c_void_p, _dp, _dp, C.POINTER(C.c_int)], C.c_int),
         "cylmhd_create": ([P, C.POINTER(C.c_void_p)], C.c_int),
         "cylmhd_destroy": ([C.c_void_p], C.c_int),
         "cylmhd_dims": ([C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
@@ -217,6 +218,24 @@ def vtou(p, V):
     U = np.zeros_like(V)
     check(lib().cylmhd_vtou_host(C.byref(p), np.ascontiguousarray(V), U), "vtou")
     return U
+
+
+def check_host(p, U, V, rho_init=None):
+    """Check (check.h:23-188) on host arrays U, V in place, as RestartRead
+    runs it; rho_init = the SaveInitial density (floors rho_init * rhocr) or
+    None (no SaveInitial yet: no density floor).  Returns the three counts."""
+    for a in (U, V):
+        if not (a.dtype == np.float64 and a.flags.c_contiguous):
+            raise ValueError("check_host works in place on C-contiguous FP64 arrays")
+    ri = None
+    if rho_init is not None:
+        ri = np.ascontiguousarray(rho_init, dtype=np.float64)
+        if ri.size != U[0].size:
+            raise ValueError("rho_init must have one value per cell")
+    c = (C.c_int * 3)()
+    check(lib().cylmhd_check_host(C.byref(p), None if ri is None else ri.ctypes.data_as(C.c_void_p), U, V, c),
+          "check_host")
+    return [c[0], c[1], c[2]]
 
 
 def binary_output(dirname, t, x, y, z, V):

@@ -12,8 +12,15 @@
  *   cylmhd_run [--config N] [--grid NR NPHI NZ] [--path plain64|fused64|fused32]
  *              [--rk 2|3] [--nstop N] [--tint T] [--tstop T] [--restart NOUT]
  *              [--outdir DIR] [--quiet] [--gdl] [--idl ROOT] [--convert FIRST LAST]
+ *              [--param NAME VALUE]...
  * --config 0 (default) is the reference's own set-up: input.h grid and
- * GalacticDisc3.  With every dump: --gdl also writes GDLOutputCylinder's
+ * GalacticDisc3.  --param sets one of the run-time parameters that input.h
+ * and constant.h fix at compile time in the reference (cfl, dtmin, rhocr,
+ * prcr, betacr, cr, rin, epssp0, tausp, omegasp, omegapattern, pitch).
+ * --restart NOUT is main.c:244-288: RestartRead(0) -- the nine primitive
+ * files of DIR/0, VtoU, Check with the floors not yet set -- then
+ * SaveInitial, then RestartRead(NOUT) with the floors of that initial state,
+ * t = NOUT*tint, tout = (NOUT+1)*tint.  With every dump: --gdl also writes GDLOutputCylinder's
  * text files (output.h:280-384; main.c:371-372 has the call commented) into
  * DIR, --idl the text slices the reference's IDL scripts read into
  * ROOT/idl<nout>/ and ROOT/azimuth-K/ (the arm potential at the dump time;
@@ -23,6 +30,7 @@
  * library error.
  */
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -55,6 +63,47 @@ static int same_coords(const char *dir, const char *name, const double *v, int n
 static void dirname_of(char *buf, size_t n, const char *outdir, int nout)
 {
     snprintf(buf, n, "%s/%d", outdir, nout);
+}
+
+/* --param NAME VALUE: the double-valued fields a user of the reference
+ * edits in input.h / constant.h */
+static int set_param(cylmhd_params *p, const char *name, const char *value)
+{
+    static const struct { const char *name; size_t off; } tab[] = {
+        {"cfl", offsetof(cylmhd_params, cfl)}, {"dtmin", offsetof(cylmhd_params, dtmin)},
+        {"rhocr", offsetof(cylmhd_params, rhocr)}, {"prcr", offsetof(cylmhd_params, prcr)},
+        {"betacr", offsetof(cylmhd_params, betacr)}, {"cr", offsetof(cylmhd_params, cr)},
+        {"rin", offsetof(cylmhd_params, rin)}, {"epssp0", offsetof(cylmhd_params, epssp0)},
+        {"tausp", offsetof(cylmhd_params, tausp)}, {"omegasp", offsetof(cylmhd_params, omegasp)},
+        {"omegapattern", offsetof(cylmhd_params, omegapattern)}, {"pitch", offsetof(cylmhd_params, pitch)},
+    };
+    char *end;
+    const double v = strtod(value, &end);
+    if (end == value || *end) return 0;
+    for (size_t q = 0; q < sizeof tab / sizeof tab[0]; q++)
+        if (!strcmp(tab[q].name, name)) {
+            *(double *)((char *)p + tab[q].off) = v;
+            return 1;
+        }
+    return 0;
+}
+
+/* RestartRead(num) (restartread.h:9-143): the nine primitive files, then
+ * VtoU and Check; rho_init = NULL while no SaveInitial has set the floors */
+static int restart_read(const cylmhd_params *p, const char *dir, int num, int NX, int NY, int NZ,
+                        const double *rho_init, double *U, double *V)
+{
+    int rc, counts[3];
+    if ((rc = cylmhd_restart_read(dir, NX, NY, NZ, V))) {
+        printf("I can't open %s.\n", dir);
+        return fail(dir, rc);
+    }
+    printf("%d binary read complete.\n", num);
+    if ((rc = cylmhd_vtou_host(p, V, U))) return fail("vtou", rc);
+    if ((rc = cylmhd_check_host(p, rho_init, U, V, counts))) return fail("check", rc);
+    printf("negative density point : %d\nnegative pressure point : %d\ncritical beta point : %d\n", counts[0],
+           counts[1], counts[2]);
+    return 0;
 }
 
 /* ZLeftVzAntisymBSymBoundary then ZLeftFreeBoundary on U (model.h:312-313,
@@ -93,6 +142,8 @@ int main(int argc, char **argv)
     int config = 0, nr = 0, nphi = 0, nz = 0, path = -1, rk = 0, restart = -1, quiet = 0, gdl = 0;
     int conv_first = -1, conv_last = -1;
     const char *idl = NULL;
+    const char *pname[32], *pval[32];
+    int nparam = 0;
     long nstop = 10000000;
     double tint = 0.05, tstop = 40.0;                      /* input.h:82-84 */
     const char *outdir = ".";
@@ -114,6 +165,7 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[a], "--gdl")) gdl = 1;
         else if (!strcmp(argv[a], "--idl") && a + 1 < argc) idl = argv[++a];
         else if (!strcmp(argv[a], "--convert") && a + 2 < argc) { conv_first = atoi(argv[++a]); conv_last = atoi(argv[++a]); }
+        else if (!strcmp(argv[a], "--param") && a + 2 < argc && nparam < 32) { pname[nparam] = argv[++a]; pval[nparam++] = argv[++a]; }
         else { fprintf(stderr, "usage: see the header of host/main.c\n"); return 2; }
     }
     if (path == -2) { fprintf(stderr, "unknown --path\n"); return 2; }
@@ -123,6 +175,8 @@ int main(int argc, char **argv)
     if (config == 0 && nr > 0) { p.ix = nr; p.jy = nphi; p.kz = nz; p.dy0 = (2.0 * M_PI) / (double)nphi; }
     if (path >= 0) p.path = path;
     if (rk) p.rk = rk;
+    for (int q = 0; q < nparam; q++)
+        if (!set_param(&p, pname[q], pval[q])) { fprintf(stderr, "--param %s %s: unknown name or bad value\n", pname[q], pval[q]); return 2; }
     const int NX = p.ix + 2, NY = p.jy + 2, NZ = p.kz + 2;
     const size_t N = (size_t)NX * NY * NZ;
     double *x = malloc(sizeof(double) * NX), *y = malloc(sizeof(double) * NY), *z = malloc(sizeof(double) * NZ);
@@ -157,15 +211,17 @@ int main(int argc, char **argv)
         return 0;
     }
     if (restart >= 0) {
-        /* main.c:250-270 */
+        /* main.c:250-270: RestartRead(0), SaveInitial, RestartRead(nout) */
+        double *rho0 = malloc(sizeof(double) * N);
+        if (!rho0) { fprintf(stderr, "out of host memory\n"); return 1; }
         dirname_of(dir, sizeof dir, outdir, 0);
-        if ((rc = cylmhd_restart_read(dir, NX, NY, NZ, V))) return fail(dir, rc);
-        if ((rc = cylmhd_vtou_host(&p, V, U))) return fail("vtou", rc);
+        if (restart_read(&p, dir, 0, NX, NY, NZ, NULL, U, V)) return 1;
         if ((rc = cylmhd_set_initial(ctx, U, V))) return fail("set_initial", rc);
+        memcpy(rho0, U, sizeof(double) * N);            /* rhofloor = U[0]*rhocr (saveinitial.h:44) */
         nout = restart;
         dirname_of(dir, sizeof dir, outdir, nout);
-        if ((rc = cylmhd_restart_read(dir, NX, NY, NZ, V))) return fail(dir, rc);
-        if ((rc = cylmhd_vtou_host(&p, V, U))) return fail("vtou", rc);
+        if (restart_read(&p, dir, nout, NX, NY, NZ, rho0, U, V)) return 1;
+        free(rho0);
         if ((rc = cylmhd_upload_state(ctx, U, V))) return fail("upload", rc);
         t = nout * tint;
         tout = (nout + 1) * tint;

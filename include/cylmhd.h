@@ -127,6 +127,13 @@ int cylmhd_fill_ghosts_host(const cylmhd_params *p, double *U, double *V);
 int cylmhd_model_galactic_disc3(const cylmhd_params *p, double *V);
 /* VtoU (vtou.h:20-62) on host arrays. */
 int cylmhd_vtou_host(const cylmhd_params *p, const double *V, double *U);
+/* Check (check.h:23-188) on host arrays, in place, as RestartRead runs it
+ * after its VtoU (restartread.h:132-139).  rho_init = the density U[0] of
+ * the state SaveInitial saved (rhofloor = rho_init * rhocr, saveinitial.h:44),
+ * or NULL before any SaveInitial (RestartRead(0), main.c:250: the
+ * reference's rhofloor is then still all zero).  counts (optional) = the
+ * density, pressure and beta floor hits (check.h:180-183). */
+int cylmhd_check_host(const cylmhd_params *p, const double *rho_init, double *U, double *V, int counts[3]);
 
 /* ---- device context: the hot path --------------------------------------- */
 typedef struct cylmhd_ctx cylmhd_ctx;
@@ -141,7 +148,8 @@ int cylmhd_dims(const cylmhd_ctx *c, int *nx, int *ny, int *nz);
  * Uinitial/Vinitial/rhofloor references of the absorbing layers and floors. */
 int cylmhd_set_initial(cylmhd_ctx *c, const double *U, const double *V);
 /* Replace the current state (restart: RestartRead restartread.h:9-143 after
- * its VtoU + Check, or any state the caller holds). */
+ * its VtoU + Check -- cylmhd_vtou_host + cylmhd_check_host -- or any state
+ * the caller holds). */
 int cylmhd_upload_state(cylmhd_ctx *c, const double *U, const double *V);
 /* Copy the current state back (V feeds BinaryOutput, output.h:1202-1414). */
 int cylmhd_download_state(cylmhd_ctx *c, double *U, double *V);

@@ -1155,6 +1155,36 @@ static void check(orc *o, double *dU, double *dV, int *counts)
     counts[0] = rhoflag; counts[1] = prflag; counts[2] = oneobetaflag;
 }
 
+/* RestartRead's tail (restartread.h:132-139): VtoU (vtou.h:20-62) of the
+ * primitives just read, then Check(U, V) with the current rhofloor -- still
+ * zero before the first orc_set_initial, as the reference's zero-initialised
+ * global is at RestartRead(0) (main.c:250).  The result is the current state. */
+void orc_restart_state(orc *o, const double *V, int counts[3])
+{
+    const size_t N = o->N;
+    const double oneogm1 = o->p.oneogm1;
+    memcpy(o->V, V, sizeof(double) * 9 * N);
+    double *U = o->U, *W = o->V;
+    for (int i = 0; i < o->NX; i++)
+        for (size_t jk = 0; jk < (size_t)o->NY * o->NZ; jk++) {
+            const size_t c = (size_t)i * o->NY * o->NZ + jk;
+            const double v2 = W[1 * N + c] * W[1 * N + c] + W[2 * N + c] * W[2 * N + c] + W[3 * N + c] * W[3 * N + c];
+            const double b2 = W[4 * N + c] * W[4 * N + c] + W[5 * N + c] * W[5 * N + c] + W[6 * N + c] * W[6 * N + c];
+            U[c] = W[c];
+            U[1 * N + c] = W[c] * W[1 * N + c];
+            U[2 * N + c] = o->x[i] * W[c] * W[2 * N + c];
+            U[3 * N + c] = W[c] * W[3 * N + c];
+            U[4 * N + c] = W[4 * N + c];
+            U[5 * N + c] = W[5 * N + c];
+            U[6 * N + c] = W[6 * N + c];
+            U[7 * N + c] = 0.5 * (W[c] * v2 + b2) + oneogm1 * W[7 * N + c];
+            U[8 * N + c] = W[8 * N + c];
+        }
+    int cnt[3];
+    check(o, U, W, cnt);
+    if (counts) { counts[0] = cnt[0]; counts[1] = cnt[1]; counts[2] = cnt[2]; }
+}
+
 /* one RK stage: FirstStep (firststep.h:19-176), SecondStep (secondstep.h:
  * 21-190), ThirdStep (thirdstep.h:19-221), Halfstep (halfstep.h:22-168),
  * Fullstep (fullstep.h:21-195) */

@@ -39,6 +39,10 @@ void   orc_gravity_axi(const orc *o, double *grx, double *grz);
  * Vinitial and rhofloor. */
 void   orc_set_initial(orc *o, const double *U, const double *V);
 void   orc_set_state(orc *o, const double *U, const double *V);
+/* RestartRead's tail (restartread.h:132-139): U = VtoU(V), then Check(U, V)
+ * with the current rhofloor (zero before orc_set_initial); becomes the
+ * current state.  counts (optional) = Check's density/pressure/beta hits. */
+void   orc_restart_state(orc *o, const double *V, int counts[3]);
 void   orc_get_state(const orc *o, double *U, double *V);
 /* Setdt (setdt.h:24-94) */
 void   orc_setdt(orc *o, double *dt, double *ch, int ijk[3]);

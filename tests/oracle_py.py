@@ -38,6 +38,7 @@ def _load(params_cls):
         "orc_gravity_axi": ([vp, _dp, _dp], None),
         "orc_set_initial": ([vp, _dp, _dp], None),
         "orc_set_state": ([vp, _dp, _dp], None),
+        "orc_restart_state": ([vp, _dp, C.POINTER(C.c_int)], None),
         "orc_get_state": ([vp, _dp, _dp], None),
         "orc_setdt": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int)], None),
         "orc_gravity": ([vp, C.c_double], None),
@@ -121,6 +122,13 @@ class Oracle:
 
     def set_state(self, U, V):
         self.L.orc_set_state(self.h, np.ascontiguousarray(U), np.ascontiguousarray(V))
+
+    def restart_state(self, V):
+        """RestartRead's tail (restartread.h:132-139): VtoU of V, then Check
+        with the current floors; returns the three Check counts."""
+        c = (C.c_int * 3)()
+        self.L.orc_restart_state(self.h, np.ascontiguousarray(V, dtype=np.float64), c)
+        return [c[0], c[1], c[2]]
 
     def state(self):
         U = np.zeros((9,) + self.dims)

@@ -11,7 +11,10 @@ run had (RestartRead -> the same GDL bytes).
 import os
 import subprocess
 
+import numpy as np
 import pytest
+
+import cylmhd as cm
 
 from conftest import PKG
 from test_slices import read_dir
@@ -61,3 +64,53 @@ def test_driver_converter_rejects_other_grid(gpu, tmp_path):
     r = subprocess.run([EXE, "--config", "1", "--grid", "18", "16", "10", "--outdir", str(out), "--convert", "0",
                         "0", "--idl", str(tmp_path / "idl")], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "grid" in r.stderr
+
+
+def test_driver_restart_matches_oracle(gpu, tmp_path):
+    """--restart NOUT (main.c:244-288): RestartRead(0) -- VtoU and Check with
+    no floors set yet --, SaveInitial, RestartRead(NOUT) -- VtoU and Check
+    with the saved floors --, t = NOUT*tint, then the loop.  The dumps carry
+    a cell below each floor so that both Checks fire; the spiral amplitude
+    ramps with t (epssp0 t / tausp, spiralgravity.h:41-42), so a wrong restart
+    time changes the state.  On the plain FP64 path every dump written after
+    the restart equals the oracle's state to 1e-12 relative per variable (the
+    device's sin/cos of the arm phase may differ from glibc's in the last
+    bit; with no spiral the two are bit-identical, test_gpu_parity), with
+    its time."""
+    from cases import rel_err
+    from oracle_py import Oracle, parse_counts
+    from test_host import restart_states
+
+    out = tmp_path / "dumps"
+    out.mkdir()
+    p = cm.params_config(1, 16, 16, 10)
+    p.epssp0, p.tausp = 0.05, 1.0
+    V0, Vk = restart_states(p)
+    x, y, z = cm.grid(p)
+    nout, tint = 3, 1.0e-3
+    cm.binary_output(str(out / "0"), 0.0, x, y, z, V0)
+    cm.binary_output(str(out / str(nout)), nout * tint, x, y, z, Vk)
+    r = run(["--config", "1", "--grid", "16", "16", "10", "--path", "plain64", "--outdir", str(out),
+             "--restart", str(nout), "--nstop", "2", "--tint", str(tint), "--tstop", "1e9",
+             "--param", "epssp0", "0.05", "--param", "tausp", "1.0"])
+    o = Oracle(p)
+    c0 = o.restart_state(cm.restart_read(str(out / "0"), p.dims))
+    U0, V0c = o.state()
+    o.set_initial(U0, V0c)
+    ck = o.restart_state(cm.restart_read(str(out / str(nout)), p.dims))
+    counts = parse_counts(r.stdout)
+    assert counts[0].tolist() == c0 and counts[1].tolist() == ck, (counts[:2], c0, ck)
+    assert ck[0] >= 1 and ck[1] >= 1 and ck[2] >= 1
+    t = nout * tint                                   # main.c:269
+    for n in range(3):
+        dt, ch, _ = o.setdt()
+        o.gravity(t)
+        o.step(dt, ch)
+        t += dt
+        d = out / str(nout + 1 + n)
+        Vd = cm.restart_read(str(d), p.dims)
+        _, Vo = o.state()
+        e = rel_err(Vd, Vo)
+        print("dump", nout + 1 + n, "max rel err", e.max(), "t", t)
+        assert np.isfinite(Vd).all() and e.max() <= 1e-12, e
+        assert (d / "GridTime.dat").read_text().split("\t")[3] == "%.3e" % t

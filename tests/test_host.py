@@ -127,3 +127,58 @@ def test_no_device_is_an_error_not_a_fallback():
     p = cm.params_config(1, 16, 16, 10)
     with pytest.raises(cm.CylmhdError):
         cm.Context(p)
+
+
+def restart_states(p):
+    """Primitives as two dumps would hold them: dump 0 (the initial state, one
+    cell below the pressure floor) and a later dump with one cell below each of
+    the three floors of Check (check.h:51-173)."""
+    _, V0 = cm.disc_state(p)
+    V0 = V0.copy()
+    V0[7, 3, 4, 5] = 0.1 * p.prcr * V0[0, 3, 4, 5]                  # p < prcr rho
+    Vk = V0.copy()
+    Vk[2] *= 1.01
+    Vk[0, 5, 6, 4] = 1.0e-2 * p.rhocr * V0[0, 5, 6, 4]              # rho < rhocr rho_init
+    Vk[7, 6, 2, 3] = 0.2 * p.prcr * Vk[0, 6, 2, 3]                  # p < prcr rho
+    Vk[5, 8, 9, 6] = math.sqrt(2.0 * Vk[7, 8, 9, 6] * 30.0)         # 1/beta = 30 > 1/betacr
+    return V0, Vk
+
+
+def test_check_host_is_the_oracle_check_at_restart():
+    """cylmhd_check_host after cylmhd_vtou_host = RestartRead's VtoU + Check
+    (restartread.h:132-139), bit for bit against the oracle's: first with the
+    floors not yet set (RestartRead(0) before SaveInitial, main.c:250), then
+    with the floors of the saved initial state (RestartRead(nout))."""
+    p = cm.params_config(1, 16, 16, 10)
+    V0, Vk = restart_states(p)
+    o = Oracle(p)
+    c0 = o.restart_state(V0)
+    Uo0, Vo0 = o.state()
+    V = V0.copy()
+    U = cm.vtou(p, V)
+    assert cm.check_host(p, U, V) == c0
+    # no density floor before SaveInitial; the planted cell and the axis
+    # ghosts (i = 0: the across-axis copy of r rho vphi at the ghost's own
+    # radius leaves p < 0 on this annulus grid) take the pressure floor
+    assert c0[0] == 0 and c0[1] >= 1 and c0[2] == 0
+    assert V[7, 3, 4, 5] == V[0, 3, 4, 5] * p.prcr
+    np.testing.assert_array_equal(U, Uo0)
+    np.testing.assert_array_equal(V, Vo0)
+    o.set_initial(Uo0, Vo0)
+    ck = o.restart_state(Vk)
+    Uok, Vok = o.state()
+    V = Vk.copy()
+    U = cm.vtou(p, V)
+    assert cm.check_host(p, U, V, rho_init=Uo0[0]) == ck
+    assert ck[0] >= 1 and ck[1] >= 1 and ck[2] >= 1
+    np.testing.assert_array_equal(U, Uok)
+    np.testing.assert_array_equal(V, Vok)
+    # the branch table of check.h: the density floor sets rho = rhofloor and
+    # zeroes vr, vz; the pressure floor sets p = prcr rho; the beta floor
+    # scales rho and p by betacr / beta
+    c = (slice(None), 5, 6, 4)
+    assert V[0][c[1:]] == Uo0[0][c[1:]] * p.rhocr and V[1][c[1:]] == 0.0 and V[3][c[1:]] == 0.0
+    assert U[2][c[1:]] == cm.grid(p)[0][5] * V[0][c[1:]] * V[2][c[1:]]
+    assert V[7, 6, 2, 3] == V[0, 6, 2, 3] * p.prcr
+    beta = V[7, 8, 9, 6] / (0.5 * (V[4, 8, 9, 6] ** 2 + V[5, 8, 9, 6] ** 2 + V[6, 8, 9, 6] ** 2))
+    assert abs(beta - p.betacr) <= 1e-14
