@@ -17,6 +17,8 @@
 #ifndef CYLMHD_DEVICE_MATH_H
 #define CYLMHD_DEVICE_MATH_H
 
+#include <climits>
+
 #include <hip/hip_runtime.h>
 
 namespace cyl {
@@ -52,6 +54,17 @@ __device__ __noinline__ double powerlaw_pow(double temperature, double index)
     return pow(temperature, index);
 }
 
+/* Cooling() is declared int (cooling.h:22), so its double result goes
+ * through a conversion to int.  C leaves that undefined for NaN (the
+ * temperature exactly at a break point makes sign = 0/0) and out of range;
+ * the compiled reference on x86-64 (cvttsd2si) gives INT_MIN there, and so
+ * does this, where the device's own conversion would give 0 for NaN. */
+template <typename T>
+__device__ __forceinline__ int ref_int(T v)
+{
+    return (v != v || !(fabs(v) < T(2147483648.0))) ? INT_MIN : (int)v;
+}
+
 /* ---- Cooling (cooling.h:22-60) --------------------------------------- */
 template <typename T>
 __device__ __forceinline__ T cooling(const Consts &c, T rho, T pr)
@@ -73,7 +86,7 @@ __device__ __forceinline__ T cooling(const Consts &c, T rho, T pr)
         if (fa == T(0.0)) return T(0.0);
         const T A = T(raymond_A(double(T(c.te0)), double(temperature)));
         /* the reference returns through int */
-        return T((int)(T(0.5) * (A * fa + B * (T(1.0) - signb * signc) + C * (T(1.0) - signc * signd))));
+        return T(ref_int(T(0.5) * (A * fa + B * (T(1.0) - signb * signc) + C * (T(1.0) - signc * signd))));
     }
     if (c.cooling == 2) {
         const T temperature = pr / rho;

@@ -106,6 +106,18 @@ def rand_case(seed=7, consistent=False, degenerate="near", cooling_band=False, *
     return p, Ui, Vi, Uc, Vc
 
 
+def break_case(cell=(6, 3, 3), temperature=0.031, **over):
+    """The random state (Raymond cooling on) with one cell at exactly a
+    break temperature of Cooling (cooling.h:31: a = 0.031): rho = 1 and
+    pr = 0.031, so T - a = 0 and sign = 0/0 = NaN (cooling.h:45); Cooling
+    returns that NaN through int (the x86-64 conversion gives INT_MIN)."""
+    p, Ui, Vi, Uc, Vc = rand_case(**over)
+    Vc[0][cell] = 1.0
+    Vc[7][cell] = temperature
+    Uc = cm.vtou(p, Vc)
+    return p, Ui, Vi, Uc, Vc
+
+
 def rel_err(a, b):
     """max |a-b| / max |b| per component (L-infinity relative)."""
     out = []

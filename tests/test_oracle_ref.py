@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import cylmhd as cm
-from cases import rand_case, ref_disc_case
+from cases import break_case, rand_case, ref_disc_case
 from oracle_py import Oracle, parse_counts, parse_dt_mesh, ref_harness, run_ref
 
 SCHEMES = {"mm_hlld": (cm.LIMITER_MINMOD, cm.SOLVER_HLLD), "mc_hlld": (cm.LIMITER_MC, cm.SOLVER_HLLD),
@@ -123,6 +123,29 @@ def test_exact_degenerate_hlld_is_nan_in_both():
     assert np.isnan(r["U"]).any()
     np.testing.assert_array_equal(np.isnan(r["U"]), np.isnan(U))
     np.testing.assert_array_equal(r["U"], U)
+
+
+def test_cooling_break_temperature_matches_reference():
+    """A cell exactly at Cooling's break temperature a = 0.031: the sign
+    factors are 0/0 = NaN and Cooling returns NaN through int, which the
+    compiled reference (x86-64) turns into INT_MIN -- a huge heating term in
+    SpiralGravityCooling's energy source.  The oracle gives the same bits."""
+    name = "rand16_mm_hlld"
+    if ref_harness(name) is None:
+        pytest.skip("reference harness not built")
+    p, Ui, Vi, Uc, Vc = break_case()
+    assert cm.lib() and Vc[7][6, 3, 3] / Vc[0][6, 3, 3] == 0.031
+    o, U, V, hist = run_oracle(p, Ui, Vi, Uc, Vc, 1)
+    gx, gz = o.gravity_axi()
+    r, out = run_ref(name, Vi, Vc, gx, gz, 1, p.rk, galactic=True, U_init=Ui, U_cur=Uc)
+    np.testing.assert_array_equal(r["U"], U)
+    np.testing.assert_array_equal(r["V"], V)
+    # the break path was taken: the energy of that cell differs from the same
+    # step with the temperature one ulp away from the break point
+    q, Ui2, Vi2, Uc2, Vc2 = break_case(temperature=np.nextafter(0.031, 1.0))
+    _, U2, _, _ = run_oracle(q, Ui2, Vi2, Uc2, Vc2, 1)
+    e, e2 = U[7][6, 3, 3], U2[7][6, 3, 3]
+    assert np.isfinite(e2) and (not np.isfinite(e) or abs(e) > 1e3 * abs(e2)), (e, e2)
 
 
 def test_geometry_and_tables_match_reference():
