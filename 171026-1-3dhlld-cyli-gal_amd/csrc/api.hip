@@ -86,20 +86,22 @@ int dupload(cylmhd_ctx *c, const T **dst, const T *src, size_t count)
     return CYLMHD_OK;
 }
 
-Consts make_consts(const cylmhd_ctx *c, double dt, double ch)
+Consts consts_of(const cylmhd_params *p, double dt, double ch)
 {
     Consts k{};
-    k.gm = c->p.gm; k.gm1 = c->p.gm1; k.oneogm1 = c->p.oneogm1;
+    k.gm = p->gm; k.gm1 = p->gm1; k.oneogm1 = p->oneogm1;
     k.dt = dt; k.ch = ch;
-    k.psicoef = exp(-dt * ch * (1.0 / c->p.cr) * 0.5);   /* tvdrungekutta.h:22 */
-    k.prcr = c->p.prcr; k.betacr = c->p.betacr; k.oneobetacr = (1.0 / c->p.betacr); k.rhocr = c->p.rhocr;
-    k.omegasp = c->p.omegasp;
-    k.te0 = c->p.te0;
-    k.cooling = c->p.cooling;
-    k.cool_lambda0 = c->p.cool_lambda0; k.cool_tmin = c->p.cool_tmin; k.cool_index = c->p.cool_index;
-    k.source = c->p.source;
+    k.psicoef = exp(-dt * ch * (1.0 / p->cr) * 0.5);   /* tvdrungekutta.h:22 */
+    k.prcr = p->prcr; k.betacr = p->betacr; k.oneobetacr = (1.0 / p->betacr); k.rhocr = p->rhocr;
+    k.omegasp = p->omegasp;
+    k.te0 = p->te0;
+    k.cooling = p->cooling;
+    k.cool_lambda0 = p->cool_lambda0; k.cool_tmin = p->cool_tmin; k.cool_index = p->cool_index;
+    k.source = p->source;
     return k;
 }
+
+Consts make_consts(const cylmhd_ctx *c, double dt, double ch) { return consts_of(&c->p, dt, ch); }
 
 int validate(const cylmhd_params *p)
 {
@@ -733,3 +735,81 @@ int cylmhd_idl_output(cylmhd_ctx *c, const char *root, int nout, const cylmhd_id
 }
 
 }  // extern "C"
+
+/* ---- Setdt with no allocation ------------------------------------------
+ * The block partials of the CFL maximum live in `scratch`, a file-scope
+ * device array: each GPU the library runs on holds its own copy until the
+ * process ends.  Host code never passes the array's name to a kernel (on the
+ * host it only stands for the symbol): the device address of each GPU's copy
+ * is looked up once per device, with that device current, and used from
+ * then on.  Every call on a GPU shares that GPU's array; calls that overlap
+ * on one GPU race. */
+#include <mutex>
+
+namespace {
+constexpr int kNoallocBlocks = 1024;                 /* partials: (max, index) per block */
+static __device__ __align__(256) double scratch[2 * kNoallocBlocks];
+constexpr int kMaxDevices = 64;
+double *g_scratch_ptr[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+int scratch_ptr(double **out)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        snprintf(g_errmsg, sizeof g_errmsg, "device %d: at most %d devices", dev, kMaxDevices);
+        return CYLMHD_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CYLMHD_OK;
+}
+}  // namespace
+
+extern "C" int cylmhd_setdt_noalloc(const cylmhd_params *p, const double *V, const double *minlength,
+                                    double minminlength, void *stream, double *dt, double *ch, int ijk[3])
+{
+    if (!p || !V || !minlength) return CYLMHD_EINVAL;
+    g_errmsg[0] = 0;
+    if (p->ix < 12 || p->jy < 1 || p->kz < 1) {
+        snprintf(g_errmsg, sizeof g_errmsg, "grid %dx%dx%d: Setdt needs ix >= 12 (it skips the outer 11 planes)",
+                 p->ix, p->jy, p->kz);
+        return CYLMHD_EINVAL;
+    }
+    const size_t NX = (size_t)p->ix + 2, NY = (size_t)p->jy + 2, NZ = (size_t)p->kz + 2;
+    if (NX * NY * NZ >= (size_t)1 << 53) {
+        /* the argmax index travels through the partials as a double */
+        snprintf(g_errmsg, sizeof g_errmsg, "grid of %zu cells: at most 2^53 cells", NX * NY * NZ);
+        return CYLMHD_EINVAL;
+    }
+    double *red = nullptr;
+    int rc = scratch_ptr(&red);
+    if (rc) return rc;
+    /* the fields of the device tables that Setdt reads */
+    DevGeom g{};
+    g.NX = (int)NX; g.NY = (int)NY; g.NZ = (int)NZ; g.jy = p->jy;
+    g.minlength = minlength;
+    const Consts k = consts_of(p, 0.0, 0.0);
+    hipStream_t s = (hipStream_t)stream;
+    launch_setdt(s, g, k, V, red, 2 * kNoallocBlocks);   /* at most kNoallocBlocks partials */
+    HIPCHK(hipGetLastError());
+    double res[2];
+    HIPCHK(hipMemcpyAsync(res, red, sizeof res, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double oneocfldt = res[0];
+    const long long ix = res[1] < 0 ? -1 : (long long)res[1];
+    if (dt) *dt = p->cfl / oneocfldt;                  /* setdt.h:84 */
+    if (ch) *ch = minminlength * oneocfldt;            /* setdt.h:89 */
+    if (ijk) {
+        ijk[0] = ix < 0 ? -1 : (int)(ix / (long long)(NY * NZ));
+        ijk[1] = ix < 0 ? -1 : (int)((ix / (long long)NZ) % (long long)NY);
+        ijk[2] = ix < 0 ? -1 : (int)(ix % (long long)NZ);
+    }
+    return CYLMHD_OK;
+}

@@ -118,6 +118,8 @@ def lib():
         "cylmhd_download_state": ([C.c_void_p, _dp, _dp], C.c_int),
         "cylmhd_setdt": ([C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                           C.POINTER(C.c_int)], C.c_int),
+        "cylmhd_setdt_noalloc": ([P, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int)], C.c_int),
         "cylmhd_gravity": ([C.c_void_p, C.c_double], C.c_int),
         "cylmhd_step": ([C.c_void_p, C.c_double, C.c_double], C.c_int),
         "cylmhd_floor_counts": ([C.c_void_p, C.POINTER(C.c_int)], C.c_int),
@@ -236,6 +238,17 @@ def check_host(p, U, V, rho_init=None):
     check(lib().cylmhd_check_host(C.byref(p), None if ri is None else ri.ctypes.data_as(C.c_void_p), U, V, c),
           "check_host")
     return [c[0], c[1], c[2]]
+
+
+def setdt_noalloc(p, V_dev, minlength_dev, minminlength, stream=None):
+    """cylmhd_setdt_noalloc on device arrays given by address (V: [9][NX][NY][NZ]
+    FP64 primitives, minlength: [NX][NZ] FP64); returns (dt, ch, (i, j, k))."""
+    dt, ch = C.c_double(), C.c_double()
+    ijk = (C.c_int * 3)()
+    check(lib().cylmhd_setdt_noalloc(C.byref(p), C.c_void_p(V_dev), C.c_void_p(minlength_dev), minminlength,
+                                     C.c_void_p(stream) if stream else None, C.byref(dt), C.byref(ch), ijk),
+          "setdt_noalloc")
+    return dt.value, ch.value, tuple(ijk)
 
 
 def binary_output(dirname, t, x, y, z, V):

@@ -163,6 +163,18 @@ int cylmhd_gravity(cylmhd_ctx *c, double t);
  * ch; RK3 = FirstStep/SecondStep/ThirdStep, RK2 = Halfstep/Fullstep.
  * Enqueued on the context stream; returns without waiting.  main.c:348. */
 int cylmhd_step(cylmhd_ctx *c, double dt, double ch);
+/* Setdt that allocates nothing, on a caller's device arrays.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture set-up, tight serving loops, harnesses that
+ * forbid hipMalloc); the result equals cylmhd_setdt's on the same state.
+ * V: device [9][NX][NY][NZ] FP64 primitives; minlength: device [NX][NZ]
+ * FP64 (cylindricalgrid.h:146); minminlength its minimum (setdt.h:89);
+ * stream: a hipStream_t to run on, or NULL.  The block partials live in a
+ * static device array of 1024 (max, index) pairs (16 KB per GPU, held until
+ * the process ends); grids of 2^53 cells or more are rejected.  Calls that
+ * overlap on one GPU race on that array.  Waits for the stream. */
+int cylmhd_setdt_noalloc(const cylmhd_params *p, const double *V, const double *minlength,
+                         double minminlength, void *stream, double *dt, double *ch, int ijk[3]);
 /* Check() floor counters (check.h:180-183) of the stages of the last step:
  * counts[3*s + 0..2] = rho, pressure, beta hits of stage s. */
 int cylmhd_floor_counts(cylmhd_ctx *c, int counts[9]);
