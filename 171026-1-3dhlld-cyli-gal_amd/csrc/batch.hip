@@ -286,14 +286,22 @@ void disc_tiling(const DiscView &v, int shape, int seglen, int &tiles_j, int &ti
     nseg = (planes + seg - 1) / seg;
 }
 
-/* work list, stage arguments and per-disc state for the current shape */
-int build(cylmhd_batch *b, bool keep_time)
+/* the per-disc step state (time, last dt, steps taken, alive) on the host */
+int snapshot(cylmhd_batch *b, std::vector<BDisc> &out)
 {
-    std::vector<BDisc> old;
-    if (keep_time && b->d_bd) {
-        old.resize(b->n);
-        BCHK(hipMemcpy(old.data(), b->d_bd, sizeof(BDisc) * b->n, hipMemcpyDeviceToHost));
-    }
+    out.resize(b->n);
+    BCHK(hipStreamSynchronize(b->stream));
+    BCHK(hipMemcpy(out.data(), b->d_bd, sizeof(BDisc) * b->n, hipMemcpyDeviceToHost));
+    return CYLMHD_OK;
+}
+
+/* work list, stage arguments and per-disc state for the current shape;
+ * keep (optional): the step state each disc resumes from (else t = 0,
+ * no steps, alive) */
+int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
+{
+    const bool keep_time = keep != nullptr;
+    const std::vector<BDisc> &old = keep_time ? *keep : std::vector<BDisc>();
     free_lists(b);
     const int n = b->n, nst = b->nst;
     std::vector<int4> items;
@@ -520,7 +528,7 @@ int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out)
             b->allocs.clear();          /* the backup outlives work-list rebuilds */
         }
     }
-    if (!rc) rc = build(b, false);
+    if (!rc) rc = build(b, nullptr);
     if (rc) {
         cylmhd_batch_destroy(b);
         return rc;
@@ -589,6 +597,13 @@ int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
     static const int seglens[4] = {8, 16, 32, 0};
     static const int grids[2] = {512, 1024};
     int rc;
+    /* every candidate starts from the discs' current states and step state
+     * (time, dt, steps, alive: the spiral ramp and arm phase follow t, and
+     * a stopped disc stays stopped and is skipped by the kernels); both are
+     * put back after each candidate and at the end */
+    std::vector<BDisc> keep;
+    if (b->built && (rc = snapshot(b, keep))) return rc;
+    const std::vector<BDisc> *kp = b->built ? &keep : nullptr;
     if ((rc = backup(b, true))) return rc;
     BCHK(hipStreamSynchronize(b->stream));
     double best = 1e300;
@@ -599,7 +614,7 @@ int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
                 b->shape = shapes[si];
                 b->seglen = seglens[li];
                 b->grid = grids[gi];
-                if ((rc = build(b, false)) || (rc = prime(b))) return rc;
+                if ((rc = build(b, kp)) || (rc = prime(b))) return rc;
                 enqueue_step(b);                      /* warm-up */
                 BCHK(hipEventRecord(b->ev0, b->stream));
                 for (int k = 0; k < steps; k++) enqueue_step(b);
@@ -612,7 +627,10 @@ int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
             }
     BCHK(hipStreamSynchronize(b->stream));
     b->shape = bs; b->seglen = bl; b->grid = bg;
-    if ((rc = build(b, false))) return rc;
+    if ((rc = build(b, kp))) return rc;
+    /* the floor counters of the timed candidates are not the discs' */
+    for (int d = 0; d < b->n; d++) BCHK(hipMemsetAsync(b->view[d].ds.counts, 0, 9 * sizeof(int), b->stream));
+    BCHK(hipStreamSynchronize(b->stream));
     if (choice) { choice[0] = bs; choice[1] = bl; choice[2] = bg; }
     return CYLMHD_OK;
 }

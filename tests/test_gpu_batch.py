@@ -91,6 +91,32 @@ def test_batch_tuner_keeps_the_states(gpu):
             assert e.max() <= 1e-6, (d, e)
 
 
+def test_batch_tuner_mid_run_keeps_time_steps_and_stopped_discs(gpu):
+    """Tuning after some steps: each disc resumes from its own time, step
+    count and alive flag (the spiral ramp and the arm phase follow t), so
+    run(3), tune(2), run(2) ends where run(5) does; a disc that stopped
+    below dtmin stays stopped.  After the tune the first dt comes from the
+    stored state instead of the stage kernel's partials, which can move it
+    in the last bits, so the states agree to FP32 rounding."""
+    ps = batch_params(dtmin=1.0)           # disc 1 stops at once
+    with start_batch(ps) as b:
+        b.run(3)
+        b.tune(2)
+        b.run(2)
+        tuned = [(b.status(d), b.disc(d).download()[0]) for d in range(len(ps))]
+    with start_batch(ps) as b:
+        b.run(5)
+        for d, p in enumerate(ps):
+            st = b.status(d)
+            st_t, U_t = tuned[d]
+            print(d, st, st_t)
+            assert st_t["alive"] == st["alive"] and st_t["steps"] == st["steps"]
+            assert st["steps"] == (0 if d == 1 else 5)
+            assert st_t["t"] == pytest.approx(st["t"], rel=1e-6, abs=0.0)
+            e = l1_rel_scaled(U_t, b.disc(d).download()[0], cm.grid(p)[0])
+            assert e.max() <= 1e-6, (d, e)
+
+
 def compare_with_single(ps, nsteps=2):
     """Two steps: the floors of Check are discontinuous in the state, and
     with the less diffusive limiters a last-ulp difference (the device's exp
