@@ -324,7 +324,6 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
             a.N = v.ds.N;
             a.NS = v.fs->NS; a.NZP = v.fs->NZP;
             a.rhofloor = (const float *)v.fs->rhofloor;
-            a.ubar = (const float *)v.fs->ubar;
             a.Uinit = v.ds.Uinit;
             a.Vinit_b = b->lim == CYLMHD_LIMITER_VANLEER ? v.ds.Vinit_y : v.ds.Vinit_z;
             a.edge = (const float *)v.fs->edge;

@@ -21,9 +21,6 @@ struct FusedState {
      *           reads them from consecutive addresses) */
     void *Un = nullptr, *A = nullptr, *B = nullptr, *shadow = nullptr;
     void *rhofloor = nullptr;  /* [N] rhocr * Uinitial[0] */
-    /* FP32 state only: the stored values of components 0, 2, 5, 7 are
-     * deviations from ubar [NX*NZ][4], the phi-mean of the initial state */
-    void *ubar = nullptr;
     void *edge = nullptr;      /* [2][9][NY][NZ] x-boundary face fluxes */
     double *u64_scratch = nullptr;   /* [9][N] FP64 export buffer */
     double *dtpart = nullptr;  /* Setdt partials of the last stage-3 launch */
@@ -44,7 +41,10 @@ struct FusedState {
     size_t nstate = 0;
     /* geometry packed for the stage kernel (few base pointers):
      *   itab [NX][IT] FP64  x, xm, oneox, oneodx, xabs, spiral r factors
-     *   ikg  [NX*NZ][4] T   g1, g2, g3, g5
+     *   ikg  [NX*NZ][4] T   g1, g2, g3, g5; FP32 state: [NX*NZ][8], then the
+     *                       offsets of components 0, 2, 5, 7 (the stored
+     *                       values of those are deviations from them: the
+     *                       phi-mean of the initial state)
      *   ikd  [NX*NZ][4] FP64 minlength (1/minlength in the FP32 build), Grxaxi, Grzaxi, spiral q
      *   ktab [NZ][KT] FP64  oneodz, z, spiral z factors, zabs */
     double *itab = nullptr, *ikd = nullptr, *ktab = nullptr;

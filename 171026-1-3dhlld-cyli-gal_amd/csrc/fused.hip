@@ -125,32 +125,32 @@ __device__ __forceinline__ size_t state_index(const DevGeom &g, int NZP, size_t 
  * the phi-mean of the initial state for the components with a slot; FP64:
  * none) */
 template <typename T>
-__device__ __forceinline__ double ub_of(const DevGeom &g, const T *ubar, size_t m, size_t r)
+__device__ __forceinline__ double ub_of(const DevGeom &g, const T *ikg, size_t m, size_t r)
 {
     if constexpr (sizeof(T) == 4) {
         const int sl = oslot((int)m);
         if (sl >= 0) {
             const size_t NYZ = (size_t)g.NY * g.NZ, i = r / NYZ, k = r % g.NZ;
-            return (double)ubar[(i * g.NZ + k) * NOFF + sl];
+            return (double)ikg[(i * g.NZ + k) * 8 + 4 + sl];
         }
     }
     return 0.0;
 }
 
 template <typename T>
-__global__ void k_to_state(DevGeom g, int NZP, size_t NS, bool zg, const T *__restrict__ ubar, const double *__restrict__ src,
+__global__ void k_to_state(DevGeom g, int NZP, size_t NS, bool zg, const T *__restrict__ ikg, const double *__restrict__ src,
                            T *__restrict__ dst, size_t N)
 {
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < 9 * N; q += (size_t)gridDim.x * blockDim.x) {
         const size_t m = q / N, r = q - m * N;
-        dst[state_index(g, NZP, NS, zg, m, r)] = T(src[q] - ub_of(g, ubar, m, r));
+        dst[state_index(g, NZP, NS, zg, m, r)] = T(src[q] - ub_of(g, ikg, m, r));
     }
 }
 
-/* ubar[i][k][slot] = phi-mean over the interior j of Uinitial[m][i][j][k]
- * for m = 0, 2, 5, 7, rounded to T (FP32 state only) */
+/* offsets ikg[i][k][4 + slot] = phi-mean over the interior j of
+ * Uinitial[m][i][j][k] for m = 0, 2, 5, 7, rounded to T (FP32 state only) */
 template <typename T>
-__global__ void k_ubar(DevGeom g, const double *__restrict__ Uinit, T *__restrict__ ubar, size_t N)
+__global__ void k_ubar(DevGeom g, const double *__restrict__ Uinit, T *__restrict__ ikg, size_t N)
 {
     const size_t nik = (size_t)g.NX * g.NZ;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < NOFF * nik; q += (size_t)gridDim.x * blockDim.x) {
@@ -158,7 +158,7 @@ __global__ void k_ubar(DevGeom g, const double *__restrict__ Uinit, T *__restric
         const size_t m = sl == 0 ? 0 : (sl == 1 ? 2 : (sl == 2 ? 5 : 7));
         double s = 0.0;
         for (int j = 1; j <= g.NY - 2; j++) s += Uinit[m * N + (i * g.NY + j) * g.NZ + k];
-        ubar[ik * NOFF + sl] = T(s / (double)(g.NY - 2));
+        ikg[ik * 8 + 4 + sl] = T(s / (double)(g.NY - 2));
     }
 }
 
@@ -170,7 +170,7 @@ __global__ void k_rhofloor(int NZ, int NZP, const double *__restrict__ Uinit0, T
 }
 
 template <typename T>
-__global__ void k_export(DevGeom g, int NZP, size_t NS, bool zg, Consts c, const T *__restrict__ ubar,
+__global__ void k_export(DevGeom g, int NZP, size_t NS, bool zg, Consts c, const T *__restrict__ ikg,
                          const T *__restrict__ U, double *__restrict__ U64, double *__restrict__ V64, size_t N)
 {
     const size_t NYZ = (size_t)g.NY * g.NZ;
@@ -178,7 +178,7 @@ __global__ void k_export(DevGeom g, int NZP, size_t NS, bool zg, Consts c, const
         double u[9], v[9];
 #pragma unroll
         for (int m = 0; m < 9; m++) {
-            u[m] = (double)U[state_index(g, NZP, NS, zg, m, q)] + ub_of(g, ubar, m, q);
+            u[m] = (double)U[state_index(g, NZP, NS, zg, m, q)] + ub_of(g, ikg, m, q);
             U64[m * N + q] = u[m];
         }
         utov<double>(c, u, g.oneox[q / NYZ], v);
@@ -188,7 +188,7 @@ __global__ void k_export(DevGeom g, int NZP, size_t NS, bool zg, Consts c, const
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag_t(DevGeom g, int NZP, bool zg, const T *__restrict__ ubar, const T *__restrict__ U,
+__global__ __launch_bounds__(256) void k_diag_t(DevGeom g, int NZP, bool zg, const T *__restrict__ ikg, const T *__restrict__ U,
                                                 size_t N, double *__restrict__ red)
 {
     /* N is the component stride of the padded state arrays */
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_diag_t(DevGeom g, int NZP, bool zg, con
         /* full values: the stored deviation + the offset of its (i, k) */
         auto ob = [&](int ii, int kk, int m) -> double {
             if constexpr (sizeof(T) == 4) {
-                if (oslot(m) >= 0) return (double)ubar[((size_t)ii * g.NZ + kk) * NOFF + oslot(m)];
+                if (oslot(m) >= 0) return (double)ikg[((size_t)ii * g.NZ + kk) * 8 + 4 + oslot(m)];
             }
             return 0.0;
         };
@@ -332,13 +332,9 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     fs.ktab = (double *)p;
     if ((rc = alloc(4 * nik * sizeof(double), &p))) return rc;
     fs.ikd = (double *)p;
-    if ((rc = alloc(4 * nik * sizeof(T), &fs.ikg))) return rc;
-    if (fs.fp32) {
-        /* state offsets: zero until the initial state is known (a state
-         * uploaded before cylmhd_set_initial is stored as it is) */
-        if ((rc = alloc(NOFF * nik * sizeof(T), &fs.ubar))) return rc;
-        if (hipMemset(fs.ubar, 0, NOFF * nik * sizeof(T)) != hipSuccess) return CYLMHD_EHIP;
-    }
+    /* g-factors and (FP32 state) the offsets, zero until the initial state
+     * is known: a state uploaded before cylmhd_set_initial is stored as it is */
+    if ((rc = alloc(ikg_w<T>() * nik * sizeof(T), &fs.ikg))) return rc;
     hipLaunchKernelGGL(k_pack<T>, dim3(nblocks(nik, 256, 4096)), dim3(256), 0, 0, g, fs.itab, (T *)fs.ikg, fs.ikd, fs.ktab);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return CYLMHD_EHIP;
     return 0;
@@ -352,7 +348,6 @@ static FArgs<T> make_args(FusedState &fs, const DevGeom &g, const DevState &d, c
     a.N = d.N;
     a.NS = fs.NS; a.NZP = fs.NZP;
     a.rhofloor = (const T *)fs.rhofloor;
-    a.ubar = (const T *)fs.ubar;
     a.Uinit = d.Uinit; a.Vinit_b = d.Vinit_z;
     a.edge = (const T *)fs.edge;
     a.counts = d.counts;
@@ -367,9 +362,9 @@ static FArgs<T> make_args(FusedState &fs, const DevGeom &g, const DevState &d, c
 int set_initial_tables(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, double rhocr)
 {
     hipLaunchKernelGGL(k_rhofloor<FT>, dim3(nblocks(d.N, 256, 8192)), dim3(256), 0, s, fs.NZ, fs.NZP, d.Uinit, (FT *)fs.rhofloor, d.N, rhocr);
-    if (fs.ubar) {
+    if (fs.fp32) {
         const size_t n = NOFF * (size_t)g.NX * g.NZ;
-        hipLaunchKernelGGL(k_ubar<FT>, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, g, d.Uinit, (FT *)fs.ubar, d.N);
+        hipLaunchKernelGGL(k_ubar<FT>, dim3(nblocks(n, 256, 4096)), dim3(256), 0, s, g, d.Uinit, (FT *)fs.ikg, d.N);
     }
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
@@ -378,14 +373,14 @@ int load_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &
 {
     (void)g;
     const size_t n = 9 * d.N;
-    hipLaunchKernelGGL(k_to_state<FT>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, (const FT *)fs.ubar, U64, (FT *)fs.Un, d.N);
+    hipLaunchKernelGGL(k_to_state<FT>, dim3(nblocks(n, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, (const FT *)fs.ikg, U64, (FT *)fs.Un, d.N);
     fs.dt_valid = false;
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
 int export_state(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c, double *V64)
 {
-    hipLaunchKernelGGL(k_export<FT>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, c, (const FT *)fs.ubar, (const FT *)fs.Un, fs.u64_scratch, V64, d.N);
+    hipLaunchKernelGGL(k_export<FT>, dim3(nblocks(d.N, 256, 16384)), dim3(256), 0, s, g, fs.NZP, fs.NS, fs.zg, c, (const FT *)fs.ikg, (const FT *)fs.Un, fs.u64_scratch, V64, d.N);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }
 
@@ -508,7 +503,7 @@ int diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d)
 {
     const size_t M = (size_t)(g.NX - 2) * (g.NY - 2) * (g.NZ - 2);
     const int nb = (int)nblocks(M, 256, (unsigned)(d.nred / 4));
-    hipLaunchKernelGGL(k_diag_t<FT>, dim3(nb), dim3(256), 0, s, g, fs.NZP, fs.zg, (const FT *)fs.ubar, (const FT *)fs.Un, fs.NS, d.red);
+    hipLaunchKernelGGL(k_diag_t<FT>, dim3(nb), dim3(256), 0, s, g, fs.NZP, fs.zg, (const FT *)fs.ikg, (const FT *)fs.Un, fs.NS, d.red);
     hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, d.red, nb);
     return hipGetLastError() == hipSuccess ? 0 : CYLMHD_EHIP;
 }

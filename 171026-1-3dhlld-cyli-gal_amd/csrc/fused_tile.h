@@ -94,6 +94,10 @@ __device__ __forceinline__ X ro(const X *p, size_t idx)
  * near the inner edge amplifies.  The full values are formed in FP64 where
  * they feed the pressure recovery.  FP64 state keeps no offset (ub unused). */
 constexpr int NOFF = 4;                  /* offset slots: components 0, 2, 5, 7 */
+/* the offsets of (i, k) sit in the (i, k) row of the ikg table, after its
+ * four g-factors: ikg [NX*NZ][IKW] of T = g1, g2, g3, g5 (, ub0, ub2, ub5,
+ * ub7 for FP32 state), so no further table pointer lives in the kernel */
+template <typename T> constexpr int ikg_w() { return sizeof(T) == 4 ? 8 : 4; }
 __host__ __device__ constexpr int oslot(int m) { return m == 0 ? 0 : (m == 2 ? 1 : (m == 5 ? 2 : (m == 7 ? 3 : -1))); }
 template <typename T> struct Prec;
 
@@ -176,8 +180,6 @@ template <typename T> struct FArgs {
     const T *Uin, *Un;
     T *Uout, *shadow;
     const T *rhofloor;
-    /* FP32 state offsets ubar [NX*NZ][NOFF] (FusedState::ubar); unused for FP64 */
-    const T *ubar;
     const double *Uinit;
     /* initial-state planes an interface boundary reads: Vinit_z [9][2][NX][NY]
      * (MC, ZVlrFixed) or Vinit_y [9][2][NX][NZ] (vanLeer, YVlrInitialize) */
@@ -205,7 +207,7 @@ template <typename T>
 __device__ __forceinline__ void ub_load(const FArgs<T> &a, int i, int k, T *ub)
 {
     if constexpr (sizeof(T) == 4) {
-        const float4 q = *(const float4 *)(a.ubar + ((size_t)i * a.g.NZ + k) * NOFF);
+        const float4 q = *(const float4 *)(a.ikg + ((size_t)i * a.g.NZ + k) * 8 + 4);
         ub[0] = q.x; ub[1] = q.y; ub[2] = q.z; ub[3] = q.w;
     }
 }
@@ -266,7 +268,9 @@ __global__ void k_pack(DevGeom g, double *itab, T *ikg, double *ikd, double *kta
 {
     const size_t nik = (size_t)g.NX * g.NZ;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < nik; q += (size_t)gridDim.x * blockDim.x) {
-        ikg[4 * q + 0] = T(g.g1[q]); ikg[4 * q + 1] = T(g.g2[q]); ikg[4 * q + 2] = T(g.g3[q]); ikg[4 * q + 3] = T(g.g5[q]);
+        constexpr int W = ikg_w<T>();
+        ikg[W * q + 0] = T(g.g1[q]); ikg[W * q + 1] = T(g.g2[q]); ikg[W * q + 2] = T(g.g3[q]); ikg[W * q + 3] = T(g.g5[q]);
+        for (int s = 4; s < W; s++) ikg[W * q + s] = T(0);      /* offsets: set with the initial state */
         /* FP32 build: 1/minlength (its Setdt multiplies); FP64: minlength */
         ikd[4 * q + 0] = sizeof(T) == 4 ? 1.0 / g.minlength[q] : g.minlength[q]; ikd[4 * q + 1] = g.grxaxi[q]; ikd[4 * q + 2] = g.grzaxi[q]; ikd[4 * q + 3] = g.sp_q[q];
         if (q < (size_t)g.NX) {
@@ -365,7 +369,7 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
      * the (i, k) rows ikg, ikd, cabs and the (i, j) arm phase sin/cos; per
      * work item the k rows (szs, z, 1 + a, z^2 s^2 term, z absorption) and
      * the z-state 1/dz pair */
-    __shared__ T s_ikg[2][TK][4];
+    __shared__ T s_ikg[2][TK][ikg_w<T>()];
     __shared__ double s_ikd[2][TK][4];
     __shared__ double s_cabs[2][TK];
     __shared__ double s_sc[2][TJ][2];
@@ -460,7 +464,9 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         if (u >= 0 && u < TK) {
             const size_t ik = (size_t)p * g.NZ + min(k0 + u, g.NZ - 1);
 #pragma unroll
-            for (int q = 0; q < 4; q++) { s_ikg[sl][u][q] = ro(a.ikg, 4 * ik + q); s_ikd[sl][u][q] = ro(a.ikd, 4 * ik + q); }
+            for (int q = 0; q < 4; q++) s_ikd[sl][u][q] = ro(a.ikd, 4 * ik + q);
+#pragma unroll
+            for (int q = 0; q < ikg_w<T>(); q++) s_ikg[sl][u][q] = ro(a.ikg, ikg_w<T>() * ik + q);
             if (p <= g.irin) s_cabs[sl][u] = ro(g.cabs, ik);
         } else if (u >= TK && u < TK + TJ) {
             const int j = min(j0 + u - TK, g.NY - 1);
@@ -717,8 +723,11 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
             const bool spiral = gravity_staged(g, i, it, s_ikd[sl][ck], s_kt[ck], s_sc[sl][cj], gx, gy, gz,
                                                ALT && (c.source == 1 || c.source == 2));
             sources<T, ALT>(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
-            T ubc[NOFF];                 /* offsets of the cell (after the RK combination) */
-            ub_load(a, i, gk, ubc);
+            T ubc[NOFF];                 /* offsets of the cell, staged with the plane's rows */
+            if constexpr (sizeof(T) == 4) {
+#pragma unroll
+                for (int q = 0; q < NOFF; q++) ubc[q] = s_ikg[sl][ck][4 + q];
+            }
 #pragma unroll
             for (int m = 0; m < 9; m++) {
                 /* the empty asm keeps the (zero or loaded) Un value in its
