@@ -13,16 +13,22 @@ value     = interior cells x K / wall seconds of K steps (state resident in
             HBM; host<->device traffic is only the scalars of Setdt).
 roofline  = the fused stage kernel: algorithmic HBM bytes per launch
             (288 B per cell-update / 3 launches, FP32 state) / its average
-            duration from HIP events on the library's stream.
+            duration from HIP events on the library's stream; traffic (HBM
+            bytes per launch) and valu_issue_frac (the time its VALU
+            instructions take at one wave64 instruction per 4 cycles on each
+            of the 1024 SIMDs at 2.4 GHz, over its duration) from the
+            rocprofv3 counter summaries under profiles/.
 precision = relative L1 / Linf error per conserved variable of the FP32
             path against the plain FP64 path after 100 steps on the same
             dt sequence, with div B and floor hits (precision.py).
 batch     = the batching form: 48 uneven discs advanced together, one launch
             per RK stage over a tile work list, per-disc dt on the device,
             the step replayed from a hipGraph, launch shape autotuned.
-cpu_baseline = the CPU restatement of the reference (oracle, OpenMP on all
-            host cores) or the reference itself (oracle/_ref bench build),
-            on a bounded sample of the same disc.
+cpu_baseline = the CPU restatement of the reference (the oracle: the
+            reference's data layout and pass structure, OpenMP on all host
+            cores) on the 256x256x128 disc of the FP64 parity runs (MC +
+            HLLD, spiral 5 %), a bounded number of steps; same_grid times
+            the GPU paths on that disc in the same run.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -177,24 +183,54 @@ def run_large(cm, np, steps):
                 precision={k: prec[k] for k in keep if k in prec})
 
 
+CPU_GRID = (256, 256, 128)
+
+
 def cpu_baseline(cm, np, budget_s=15.0):
-    """CPU restatement of the reference on a bounded sample: the same disc at
-    128x128x64, as many whole steps as fit the time budget (at least 1)."""
+    """The CPU restatement of the reference on the 256x256x128 disc (MC +
+    HLLD, spiral m = 2 at 5 %), FP64 in the reference's layout: one warm-up
+    step, then as many whole steps as fit the time budget (1 to 20)."""
     from oracle_py import Oracle
-    grid = (128, 128, 64)
-    p = cm.params_config(3, *grid)
+    p = cm.params_config(2, *CPU_GRID)
     U, V = cm.disc_state(p)
     o = Oracle(p)
     o.set_initial(U, V)
+    del U, V
     threads = o.L.orc_threads()
     t, el = o.run(1)                          # first step also warms the caches
     n = max(1, min(20, int(budget_s / max(el, 1e-3))))
     t, el = o.run(n, t)
     ncell = p.ix * p.jy * p.kz
     o.close()
+    log(f"[bench] cpu oracle: {ncell * n / el:.4g} cell-updates/s on {threads} threads ({n} steps, {el:.1f} s)")
     return dict(value=ncell * n / el, unit="cell-updates/s", cores=threads, kind="port",
-                sample=f"same disc at {grid[0]}x{grid[1]}x{grid[2]} (FP64, MC+HLLD, RK3), {n} steps after 1 "
-                       f"warm-up step, oracle/liboracle.so with OpenMP on {threads} threads")
+                sample=f"256x256x128 disc (FP64, MC+HLLD, spiral 5%, RK3; the FP64 parity disc), {n} steps after "
+                       f"1 warm-up step, oracle/liboracle.so (reference layout and pass structure) with OpenMP "
+                       f"on {threads} threads")
+
+
+def same_grid(cm, np, cpu):
+    """The GPU paths on the CPU baseline's disc, in the same run."""
+    res = {"grid": list(CPU_GRID), "cpu_oracle": cpu.get("value")}
+    for path in (cm.PATH_PLAIN64, cm.PATH_FUSED64, cm.PATH_FUSED32):
+        r = run_path(cm, np, path, CPU_GRID, 10, 1, config=2)
+        res[cm.PATH_NAMES[path]] = r["value"]
+        if cpu.get("value"):
+            res[cm.PATH_NAMES[path] + "_over_cpu"] = r["value"] / cpu["value"]
+    return res
+
+
+def counters(grid):
+    """Per-launch HBM traffic and VALU instructions of the FP32 stage kernel
+    from the rocprofv3 summaries under profiles/ (None where absent)."""
+    out = {}
+    for key, name in (("traffic", "traffic"), ("valu", "counters")):
+        f = os.path.join(ROOT, "profiles", f"{name}_fused32_{grid[0]}x{grid[1]}x{grid[2]}.json")
+        if os.path.isfile(f):
+            with open(f) as fh:
+                d = json.load(fh)
+            out[key] = d.get("bytes_per_launch") if key == "traffic" else d.get("sq_insts_valu_per_launch")
+    return out
 
 
 def main():
@@ -244,15 +280,19 @@ def main():
     ncell = main_res["interior_cells"]
     bytes_launch = BYTES_PER_CELL_UPDATE["fused32"] * ncell / 3.0
     achieved = bytes_launch / (main_res["kernel_ms"] * 1e-3) if main_res["kernel_ms"] > 0 else None
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_fused32_{grid[0]}x{grid[1]}x{grid[2]}.json")
-    if os.path.isfile(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    cnt = counters(grid)
+    traffic = cnt.get("traffic")
+    valu = cnt.get("valu")
+    # VALU issue floor: one wave64 VALU instruction holds a SIMD for 4 cycles
+    # (FP32 unpacked and FP64 alike); 256 CUs x 4 SIMDs at 2.4 GHz
+    valu_floor_ms = valu * 4.0 / (1024 * 2.4e9) * 1e3 if valu else None
     roofline = dict(bound="hbm", achieved=achieved / 1e9 if achieved else None, peak=HBM_PEAK / 1e9, unit="GB/s",
                     frac=achieved / HBM_PEAK if achieved else None, traffic=traffic,
                     kernel="k_fused<float,MC,HLLD>", algorithmic_bytes_per_launch=bytes_launch,
-                    kernel_ms=main_res["kernel_ms"])
+                    kernel_ms=main_res["kernel_ms"],
+                    valu_insts_per_launch=valu, valu_issue_floor_ms=valu_floor_ms,
+                    valu_issue_frac=valu_floor_ms / main_res["kernel_ms"] if valu_floor_ms and main_res["kernel_ms"]
+                    else None)
     out = {
         "metric": METRIC,
         "value": main_res["value"],
@@ -296,6 +336,7 @@ def main():
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cm, np)
+            out["same_grid"] = same_grid(cm, np, out["cpu_baseline"])
         except Exception as e:                     # reported, never fatal to the GPU number
             out["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
