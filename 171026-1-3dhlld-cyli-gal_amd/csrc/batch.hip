@@ -14,8 +14,11 @@
  *     would stop it (main.c:331-334), while the others go on;
  *   - so a whole step needs nothing from the host: it is captured once into
  *     a hipGraph and replayed;
- *   - an autotuner over the tile shape, the r-segment length and the grid of
- *     the stage launch, timed with HIP events on the batch stream.
+ *   - an autotuner over the tile (its phi x z extents, waves per tile and
+ *     register budget), the r-segment length and the grid of the stage
+ *     launch, first for all discs together and then per class of disc size
+ *     (discs of one phi x z size share a tile; one stage launch per tile in
+ *     use), timed with HIP events on the batch stream.
  */
 #define FUSED_NS b32
 #define FUSED_FT float
@@ -53,17 +56,25 @@ __device__ __forceinline__ int find_disc(const int *__restrict__ off, int n, int
     return lo;
 }
 
+/* the work items [lo, hi) of one tile shape */
 template <int LIM, bool HLL, int S, bool ALT>
 __global__ __launch_bounds__((tile_threads<float, S>()), (tile_waves<float, S>())) void k_fused_batch(
-    const FArgs<float> *__restrict__ descs, const int4 *__restrict__ items, int nitems, const BDisc *__restrict__ bd)
+    const FArgs<float> *__restrict__ descs, const int4 *__restrict__ items, int lo, int hi, const BDisc *__restrict__ bd)
 {
     const int b = remap_block(blockIdx.x, gridDim.x);
-    for (int it = b; it < nitems; it += gridDim.x) {
+    for (int it = lo + b; it < hi; it += gridDim.x) {
         const int4 w = items[it];
         if (!bd[w.x].alive) continue;
         fused_tile<float, LIM, HLL, S, ALT, true>(descs[w.x], w.y, w.z, it);
     }
 }
+
+/* tile shapes compiled for a limiter x solver: all four for minmod and MC
+ * with HLLD, shapes 0 and 1 for the others */
+constexpr int kShapes = 4;
+constexpr bool shape_built(int lim, bool hll, int s) { return s < 2 || ((lim == 0 || lim == 1) && !hll); }
+constexpr int shape_tj(int s) { return s == 1 || s == 2 ? 6 : (s == 3 ? 10 : 14); }
+constexpr int shape_tk(int s) { return s == 1 ? 32 : 16; }
 
 template <int LIM, bool HLL>
 __global__ __launch_bounds__(256) void k_edge_batch(const FArgs<float> *__restrict__ descs, const int *__restrict__ off,
@@ -224,8 +235,12 @@ struct cylmhd_batch {
     bool alt = false;          /* some disc runs a source other than the default */
     bool hll = false, rk3 = true;
     int nst = 3;
-    /* tuned launch shape */
-    int shape = 0, seglen = 0, grid = 1024;
+    /* tuned launch shape: a tile shape per disc, the r-segment length and
+     * the stage launch's grid; the work list holds the items of one shape
+     * contiguously, groups[s] = [first, end) of shape s */
+    std::vector<int> dshape;
+    int seglen = 0, grid = 1024;
+    int groups[kShapes][2] = {};
     /* device side */
     FArgs<float> *d_descs = nullptr;
     int4 *d_items = nullptr;
@@ -277,8 +292,7 @@ void free_lists(cylmhd_batch *b)
 /* tiles and r-segments of one disc for the chosen shape */
 void disc_tiling(const DiscView &v, int shape, int seglen, int &tiles_j, int &tiles_k, int &seg, int &nseg)
 {
-    const int TJ = shape ? Tile<float, 1>::TJ : Tile<float, 0>::TJ;
-    const int TK = shape ? Tile<float, 1>::TK : Tile<float, 0>::TK;
+    const int TJ = shape_tj(shape), TK = shape_tk(shape);
     tiles_j = (v.dg.NY - 2 + TJ - 1) / TJ;
     tiles_k = (v.dg.NZ - 2 + TK - 1) / TK;
     const int planes = v.dg.NX - 2;
@@ -309,13 +323,28 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
     std::vector<BDisc> bd(n);
     std::vector<int> eoff(n + 1, 0), hoff(n + 1, 0), soff(n + 1, 0);
     b->ioff.assign(n + 1, 0);
+    std::vector<int> ilo(n), ihi(n);
+    /* the items of each shape together (discs of a shape in disc order) */
+    for (int s = 0; s < kShapes; s++) {
+        b->groups[s][0] = (int)items.size();
+        for (int d = 0; d < n; d++) {
+            if (b->dshape[d] != s) continue;
+            int tj, tk, seg, nseg;
+            disc_tiling(b->view[d], s, b->seglen, tj, tk, seg, nseg);
+            ilo[d] = (int)items.size();
+            for (int sgm = 0; sgm < nseg; sgm++)
+                for (int tile = 0; tile < tj * tk; tile++) items.push_back(make_int4(d, tile, sgm, 0));
+            ihi[d] = (int)items.size();
+        }
+        b->groups[s][1] = (int)items.size();
+    }
     for (int d = 0; d < n; d++) {
         const DiscView &v = b->view[d];
         int tj, tk, seg, nseg;
-        disc_tiling(v, b->shape, b->seglen, tj, tk, seg, nseg);
-        b->ioff[d] = (int)items.size();
-        for (int sgm = 0; sgm < nseg; sgm++)
-            for (int tile = 0; tile < tj * tk; tile++) items.push_back(make_int4(d, tile, sgm, 0));
+        disc_tiling(v, b->dshape[d], b->seglen, tj, tk, seg, nseg);
+        /* block -> disc map of k_setdt_batch: disc d's blocks are
+         * ioff[d] .. ioff[d+1]-1, one per work item of d */
+        b->ioff[d + 1] = b->ioff[d] + (ihi[d] - ilo[d]);
         const Consts c = disc_consts(b->disc[d], 0.0, 0.0);
         for (int s = 0; s < nst; s++) {
             FArgs<float> a{};
@@ -367,8 +396,7 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
         soff[d + 1] = soff[d] + (int)std::min<size_t>(cdiv((size_t)v.dg.NX * v.dg.NY, 256), 256);
     }
     b->nitems = (int)items.size();
-    b->ioff[n] = b->nitems;
-    for (int d = 0; d < n; d++) { bd[d].item_lo = b->ioff[d]; bd[d].item_hi = b->ioff[d + 1]; }
+    for (int d = 0; d < n; d++) { bd[d].item_lo = ilo[d]; bd[d].item_hi = ihi[d]; }
     b->eblocks = eoff[n]; b->hblocks = hoff[n]; b->sblocks = soff[n];
     void *p;
     int rc;
@@ -401,18 +429,27 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
     return CYLMHD_OK;
 }
 
+template <int LIM, bool HLL, int S, bool ALT>
+void launch_shape(cylmhd_batch *b, const FArgs<float> *descs)
+{
+    if constexpr (shape_built(LIM, HLL, S)) {
+        const int lo = b->groups[S][0], hi = b->groups[S][1];
+        if (hi <= lo) return;
+        const int G = std::max(8, std::min((hi - lo + 7) / 8 * 8, b->grid));
+        hipLaunchKernelGGL((k_fused_batch<LIM, HLL, S, ALT>), dim3(G), dim3(tile_threads<float, S>()), 0, b->stream,
+                           descs, b->d_items, lo, hi, b->d_bd);
+    }
+}
+
 template <int LIM, bool HLL, bool ALT>
 void launch_stage(cylmhd_batch *b, int s)
 {
     const FArgs<float> *descs = b->d_descs + (size_t)s * b->n;
     hipLaunchKernelGGL((k_edge_batch<LIM, HLL>), dim3(b->eblocks), dim3(256), 0, b->stream, descs, b->d_eoff, b->n, b->d_bd);
-    const int G = std::max(8, std::min((b->nitems + 7) / 8 * 8, b->grid));
-    if (b->shape)
-        hipLaunchKernelGGL((k_fused_batch<LIM, HLL, 1, ALT>), dim3(G), dim3(tile_threads<float, 1>()), 0, b->stream, descs,
-                           b->d_items, b->nitems, b->d_bd);
-    else
-        hipLaunchKernelGGL((k_fused_batch<LIM, HLL, 0, ALT>), dim3(G), dim3(tile_threads<float, 0>()), 0, b->stream, descs,
-                           b->d_items, b->nitems, b->d_bd);
+    launch_shape<LIM, HLL, 0, ALT>(b, descs);
+    launch_shape<LIM, HLL, 1, ALT>(b, descs);
+    launch_shape<LIM, HLL, 2, ALT>(b, descs);
+    launch_shape<LIM, HLL, 3, ALT>(b, descs);
     hipLaunchKernelGGL(k_halo_batch, dim3(b->hblocks), dim3(256), 0, b->stream, descs, b->d_hoff, b->n, b->d_bd);
 }
 
@@ -499,6 +536,7 @@ int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out)
     }
     cylmhd_batch *b = new cylmhd_batch();
     b->n = n;
+    b->dshape.assign(n, 0);
     b->lim = p[0].limiter;
     for (int d = 0; d < n; d++) b->alt = b->alt || p[d].source != CYLMHD_SOURCE_SPIRAL_COOLING;
     b->hll = p[0].solver == CYLMHD_SOLVER_HLL;
@@ -592,7 +630,6 @@ int cylmhd_batch_status(cylmhd_batch *b, int d, double *t, double *dt, int *step
 int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
 {
     if (!b || steps < 1) return CYLMHD_EINVAL;
-    static const int shapes[2] = {0, 1};
     static const int seglens[4] = {8, 16, 32, 0};
     static const int grids[2] = {512, 1024};
     int rc;
@@ -605,32 +642,81 @@ int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
     const std::vector<BDisc> *kp = b->built ? &keep : nullptr;
     if ((rc = backup(b, true))) return rc;
     BCHK(hipStreamSynchronize(b->stream));
+    auto time_it = [&](float *ms) -> int {
+        int r;
+        if ((r = build(b, kp)) || (r = prime(b))) return r;
+        enqueue_step(b);                      /* warm-up */
+        BCHK(hipEventRecord(b->ev0, b->stream));
+        for (int k = 0; k < steps; k++) enqueue_step(b);
+        BCHK(hipEventRecord(b->ev1, b->stream));
+        BCHK(hipStreamSynchronize(b->stream));
+        BCHK(hipEventElapsedTime(ms, b->ev0, b->ev1));
+        return backup(b, false);
+    };
+    std::vector<int> avail;
+    for (int sh = 0; sh < kShapes; sh++)
+        if (sh < 2 || ((b->lim == 0 || b->lim == 1) && !b->hll)) avail.push_back(sh);
+    /* 1: one shape for every disc x segment length x grid */
     double best = 1e300;
     int bs = 0, bl = 0, bg = 1024;
-    for (int si = 0; si < 2; si++)
+    for (int sh : avail)
         for (int li = 0; li < 4; li++)
             for (int gi = 0; gi < 2; gi++) {
-                b->shape = shapes[si];
+                b->dshape.assign(b->n, sh);
                 b->seglen = seglens[li];
                 b->grid = grids[gi];
-                if ((rc = build(b, kp)) || (rc = prime(b))) return rc;
-                enqueue_step(b);                      /* warm-up */
-                BCHK(hipEventRecord(b->ev0, b->stream));
-                for (int k = 0; k < steps; k++) enqueue_step(b);
-                BCHK(hipEventRecord(b->ev1, b->stream));
-                BCHK(hipStreamSynchronize(b->stream));
                 float f = 0.0f;
-                BCHK(hipEventElapsedTime(&f, b->ev0, b->ev1));
-                if (f < best) { best = f; bs = shapes[si]; bl = seglens[li]; bg = grids[gi]; }
-                if ((rc = backup(b, false))) return rc;
+                if ((rc = time_it(&f))) return rc;
+                if (f < best) { best = f; bs = sh; bl = seglens[li]; bg = grids[gi]; }
             }
+    b->dshape.assign(b->n, bs);
+    b->seglen = bl;
+    b->grid = bg;
+    /* 2: per class of disc size (the discs of one phi x z size), the other
+     * shapes, largest classes first, each kept where it is faster */
+    std::vector<std::pair<long long, long long>> classes;   /* (cells, key) */
+    for (int d = 0; d < b->n; d++) {
+        const long long key = (long long)b->view[d].dg.NY * 100000 + b->view[d].dg.NZ;
+        bool found = false;
+        for (auto &c : classes)
+            if (c.second == key) { c.first += (long long)b->view[d].dg.NX * b->view[d].dg.NY * b->view[d].dg.NZ; found = true; }
+        if (!found) classes.push_back({(long long)b->view[d].dg.NX * b->view[d].dg.NY * b->view[d].dg.NZ, key});
+    }
+    std::sort(classes.begin(), classes.end(), [](const auto &x, const auto &y) { return x.first > y.first; });
+    if (avail.size() > 1 && classes.size() > 1) {
+        for (const auto &c : classes) {
+            std::vector<int> members;
+            for (int d = 0; d < b->n; d++)
+                if ((long long)b->view[d].dg.NY * 100000 + b->view[d].dg.NZ == c.second) members.push_back(d);
+            const int cur = b->dshape[members[0]];
+            int pick = cur;
+            for (int sh : avail) {
+                if (sh == cur) continue;
+                for (int d : members) b->dshape[d] = sh;
+                float f = 0.0f;
+                if ((rc = time_it(&f))) return rc;
+                if (f < best) { best = f; pick = sh; }
+            }
+            for (int d : members) b->dshape[d] = pick;
+        }
+    }
     BCHK(hipStreamSynchronize(b->stream));
-    b->shape = bs; b->seglen = bl; b->grid = bg;
     if ((rc = build(b, kp))) return rc;
     /* the floor counters of the timed candidates are not the discs' */
     for (int d = 0; d < b->n; d++) BCHK(hipMemsetAsync(b->view[d].ds.counts, 0, 9 * sizeof(int), b->stream));
     BCHK(hipStreamSynchronize(b->stream));
     if (choice) { choice[0] = bs; choice[1] = bl; choice[2] = bg; }
+    return CYLMHD_OK;
+}
+
+int cylmhd_batch_disc_tile(cylmhd_batch *b, int d, int tile[4])
+{
+    if (!b || !tile || d < 0 || d >= b->n) return CYLMHD_EINVAL;
+    const int sh = b->dshape[d];
+    tile[0] = sh;
+    tile[1] = shape_tj(sh);
+    tile[2] = shape_tk(sh);
+    tile[3] = (shape_tj(sh) + 2) * shape_tk(sh);
     return CYLMHD_OK;
 }
 

@@ -46,6 +46,7 @@
 #define CYLMHD_FUSED_TILE_H
 
 #include <climits>
+#include <type_traits>
 
 #include "fused.h"
 
@@ -295,7 +296,8 @@ __global__ void k_pack(DevGeom g, double *itab, T *ikg, double *ikd, double *kta
  * ikd = (Grx axisymmetric, Grz axisymmetric, q) of cell (i, k), kt = the
  * k row (szs, z, 1 + a, z^2 s^2 term), sc = (sin, cos) of the arm phase
  * at (i, j) */
-__device__ __forceinline__ bool gravity_staged(const DevGeom &g, int i, cdp it, const double *ikd, const double *kt,
+template <typename TT>
+__device__ __forceinline__ bool gravity_staged(const DevGeom &g, int i, cdp it, const TT *ikd, const TT *kt,
                                                const double *sc, double &gx, double &gy, double &gz, bool axi_only)
 {
     if (i < g.irin || axi_only) {
@@ -332,9 +334,13 @@ __device__ __forceinline__ bool gravity_staged(const DevGeom &g, int i, cdp it, 
 template <typename T, int SHAPE> struct Tile;
 template <> struct Tile<float, 0>  { static constexpr int TJ = 14, TK = 16, WAVES = 2; };
 template <> struct Tile<double, 0> { static constexpr int TJ = 6, TK = 16, WAVES = 1; };
-/* a wider-in-z shape for small discs (the batch autotuner picks one) */
+/* the batch autotuner's other shapes: wider in z (1); a two-wave tile (2:
+ * 40 KB of LDS, four blocks per CU) and a three-wave tile (3: 60 KB, two
+ * blocks per CU) */
 template <> struct Tile<float, 1>  { static constexpr int TJ = 6, TK = 32, WAVES = 2; };
 template <> struct Tile<double, 1> { static constexpr int TJ = 6, TK = 16, WAVES = 1; };
+template <> struct Tile<float, 2>  { static constexpr int TJ = 6, TK = 16, WAVES = 2; };
+template <> struct Tile<float, 3>  { static constexpr int TJ = 10, TK = 16, WAVES = 2; };
 template <typename T, int SHAPE> constexpr int tile_threads() { return (Tile<T, SHAPE>::TJ + 2) * Tile<T, SHAPE>::TK; }
 template <typename T, int SHAPE> constexpr int tile_waves() { return Tile<T, SHAPE>::WAVES; }
 
@@ -370,16 +376,20 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
      * work item the k rows (szs, z, 1 + a, z^2 s^2 term, z absorption) and
      * the z-state 1/dz pair */
     __shared__ T s_ikg[2][TK][ikg_w<T>()];
-    __shared__ double s_ikd[2][TK][4];
-    __shared__ double s_cabs[2][TK];
+    /* FP64 rows; in FP32 for the two-wave FP32 tile, whose LDS then fits
+     * four blocks to a CU (gravity, 1/minlength and the absorbing factors of
+     * the FP32 path, rounded once) */
+    using TT = typename std::conditional<sizeof(T) == 4 && NTH <= 128, float, double>::type;
+    __shared__ TT s_ikd[2][TK][4];
+    __shared__ TT s_cabs[2][TK];
     __shared__ double s_sc[2][TJ][2];
-    __shared__ double s_kt[TK][5];
+    __shared__ TT s_kt[TK][5];
     __shared__ T s_odz[TK + 2][2];
     /* MC z boundary (ZVlrFixedBoundary): Vinitial at k = 0 (side 0) and
      * k = NZ-1 (side 1) for the tile's phi rows of plane i; staged at the
      * top of the plane in tiles that touch a z boundary, read by the z-face
      * pass before the update barrier, so one buffer serves */
-    __shared__ T s_vz[2][TJ][9];
+    __shared__ T s_vz[MC ? 2 : 1][MC ? TJ : 1][9];
     static_assert(TK + TJ <= NTH - NCELL, "no spare threads for the table rows");
     static_assert(2 * TJ * 9 <= NTH, "no threads for the z-boundary rows");
 

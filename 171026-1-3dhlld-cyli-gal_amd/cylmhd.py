@@ -142,6 +142,7 @@ def lib():
         "cylmhd_batch_disc": ([C.c_void_p, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
         "cylmhd_batch_tune": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "cylmhd_batch_run": ([C.c_void_p, C.c_int, C.POINTER(C.c_double)], C.c_int),
+        "cylmhd_batch_disc_tile": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "cylmhd_batch_status": ([C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
         "cylmhd_batch_floor_counts": ([C.c_void_p, C.c_int, C.POINTER(C.c_int)], C.c_int),
@@ -423,6 +424,12 @@ class Batch:
         ch = (C.c_int * 3)()
         check(lib().cylmhd_batch_tune(self.h, steps, ch), "batch_tune")
         return tuple(ch)
+
+    def disc_tile(self, d):
+        """(shape, phi cells, z cells, threads) of the tile disc d runs with."""
+        t = (C.c_int * 4)()
+        check(lib().cylmhd_batch_disc_tile(self.h, d, t), "batch_disc_tile")
+        return tuple(t)
 
     def run(self, nsteps):
         ms = C.c_double()

@@ -143,7 +143,13 @@ def run_batch(cm, steps, n=48, seed=20171026):
         U, V = cm.disc_state(p)
         b.disc(d).set_initial(U, V)
     log(f"[bench] batch: {n} discs, {cells / 1e6:.2f} M cells, set-up {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
     choice = b.tune(2)
+    tune_s = time.perf_counter() - t0
+    tiles = {}
+    for d, p in enumerate(ps):
+        sh, tj, tk, th = b.disc_tile(d)
+        tiles.setdefault(f"{p.jy}x{p.kz}", f"{tj}x{tk}/{th}")
     b.run(2)                                   # warm-up (graph capture)
     ms = b.run(steps)
     st = [b.status(d) for d in range(n)]
@@ -153,7 +159,9 @@ def run_batch(cm, steps, n=48, seed=20171026):
     updates = sum(p.ix * p.jy * p.kz * (x["steps"] - 2) for p, x in zip(ps, st))
     res = dict(discs=n, interior_cells=cells, steps=steps, ms=ms, value=updates / (ms * 1e-3),
                unit="cell-updates/s", alive=alive, steps_done=done,
-               tuned={"shape": ["14x16", "6x32"][choice[0]], "segment": choice[1] or "whole r", "grid": choice[2]})
+               tuned={"common_tile": ["14x16/256", "6x32/256", "6x16/128", "10x16/192"][choice[0]],
+                      "segment": choice[1] or "whole r", "grid": choice[2],
+                      "tile_by_class_nphi_x_nz": tiles, "tune_s": tune_s})
     log(f"[bench] batch: {res['value']:.4g} cell-updates/s over {steps} steps ({ms:.1f} ms), tuned {res['tuned']}")
     return res
 

@@ -78,7 +78,8 @@ def test_batch_tuner_keeps_the_states(gpu):
     with start_batch(ps) as b:
         choice = b.tune(2)
         print("tuned", choice)
-        assert choice[0] in (0, 1) and choice[1] in (0, 8, 16, 32) and choice[2] in (512, 1024)
+        assert choice[0] in (0, 1, 2, 3) and choice[1] in (0, 8, 16, 32) and choice[2] in (512, 1024)
+        assert all(b.disc_tile(d)[0] in (0, 1, 2, 3) for d in range(len(ps)))
         b.run(3)
         tuned = [b.disc(d).download()[0] for d in range(len(ps))]
         times = [b.status(d)["t"] for d in range(len(ps))]
