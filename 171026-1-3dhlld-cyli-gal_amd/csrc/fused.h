@@ -25,6 +25,13 @@ struct FusedState {
     double *u64_scratch = nullptr;   /* [9][N] FP64 export buffer */
     double *dtpart = nullptr;  /* Setdt partials of the last stage-3 launch */
     int dt_blocks = 0;
+    int dt_nb = 0;             /* partials the last stage wrote */
+    /* the stage as two kernels, the r-march (writing each cell's
+     * divergence into div, [9][NS]) and a per-cell update (the default for
+     * FP32 state), or as one kernel (FP64 state); CYLMHD_STAGE_KERNELS=1
+     * or 2 in the environment at context creation selects one or two */
+    bool split = false;
+    void *div = nullptr;
     bool dt_valid = false;
     int nseg = 1, seglen = 1, tiles_j = 1, tiles_k = 1;
     /* r-segment counts the first stage-1 launch times (the fastest is kept) */

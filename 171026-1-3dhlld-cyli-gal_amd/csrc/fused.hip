@@ -8,12 +8,13 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 /* This file is compiled twice: FUSED_PREC=64 (FP64 state, -ffp-contract=off
  * like the plain path) and FUSED_PREC=32 (FP32 state; contraction into FMAs
- * and the fast FP32 divide/sqrt are allowed there, its bar is a relative L1
- * error, not bit parity).  Each build defines the entry points of its own
- * namespace, cyl::f64 or cyl::f32. */
+ * within each expression and the fast FP32 divide/sqrt are allowed there,
+ * its bar is a relative L1 error, not bit parity).  Each build defines the
+ * entry points of its own namespace, cyl::f64 or cyl::f32. */
 #if FUSED_PREC == 32
 #define FUSED_NS f32
 #define FUSED_FT float
@@ -39,6 +40,33 @@ __global__ __launch_bounds__((tile_threads<T, 0>()), (tile_waves<T, 0>())) void 
         return;
     }
     fused_tile<T, LIM, HLL, 0, ALT>(a, b % a.ntiles, b / a.ntiles, blockIdx.x);
+}
+
+/* ---- the split stage (FP32 state): r-march, then the per-cell update ---- */
+/* waves per SIMD the r-march is built for: FP32 three 256-thread blocks
+ * per CU (51 KB of LDS each, <= 168 VGPRs), FP64 two waves (256 VGPRs) */
+template <typename T> constexpr int march_waves() { return sizeof(T) == 4 ? 3 : 2; }
+
+template <typename T, int LIM, bool HLL>
+__global__ __launch_bounds__((tile_threads<T, 0>()), (march_waves<T>())) void k_march(FArgs<T> a)
+{
+    const int b = remap_block(blockIdx.x, gridDim.x);
+    if (b >= a.nblocks) return;
+    march_tile<T, LIM, HLL, 0>(a, b % a.ntiles, b / a.ntiles);
+}
+
+template <typename T, bool ALT>
+__global__ __launch_bounds__(256) void k_update(FArgs<T> a, int nch)
+{
+    update_cell<T, ALT>(a, nch);
+}
+
+/* blocks of the split stage's update: four wavefronts of 64 k each */
+static int update_blocks(const DevGeom &g, int &nch)
+{
+    nch = (g.NZ - 2 + 63) / 64;
+    const long long waves = (long long)(g.NX - 2) * (g.NY - 2) * nch;
+    return (int)((waves + 3) / 4);
 }
 
 template <typename T, int LIM, bool HLL>
@@ -87,19 +115,32 @@ __global__ __launch_bounds__(256) void k_setdt_state(FArgs<T> a)
     if (threadIdx.x == 0) { a.dtpart[2 * blockIdx.x] = best; a.dtpart[2 * blockIdx.x + 1] = (double)bix; }
 }
 
-__global__ __launch_bounds__(256) void k_dt_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+/* the Setdt partials (up to one per block of the last stage's update
+ * kernel: tens of thousands on the split stage) -> dt and its cell; four
+ * partials per thread and round, loaded before any is compared, so that
+ * the loads are in flight together */
+__global__ __launch_bounds__(1024) void k_dt_final(const double *__restrict__ part, int nb, double *__restrict__ out)
 {
-    __shared__ double sv[4];
-    __shared__ long long sx[4];
+    __shared__ double sv[16];
+    __shared__ long long sx[16];
     const int t = threadIdx.x;
     double best = 0.0;
     long long bix = LLONG_MAX;
-    for (int b = t; b < nb; b += 256) {
-        const double v = part[2 * b];
-        const long long ix = part[2 * b + 1] >= 9.0e18 ? LLONG_MAX : (long long)part[2 * b + 1];
-        if (dt_better(v, ix, best, bix) && v > 0.0) { best = v; bix = ix; }
+    for (int b0 = t; b0 < nb; b0 += 4 * 1024) {
+        double2 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int b = b0 + u * 1024;
+            q[u] = b < nb ? ((const double2 *)part)[b] : make_double2(0.0, 9.3e18);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double v = q[u].x;
+            const long long ix = q[u].y >= 9.0e18 ? LLONG_MAX : (long long)q[u].y;
+            if (dt_better(v, ix, best, bix) && v > 0.0) { best = v; bix = ix; }
+        }
     }
-    block_dt_max<4>(best, bix, sv, sx);
+    block_dt_max<16>(best, bix, sv, sx);
     if (t == 0) { out[0] = best; out[1] = bix == LLONG_MAX ? -1.0 : (double)bix; }
 }
 
@@ -281,6 +322,11 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     void *p;
     if ((rc = alloc(9 * N * sizeof(double), &p))) return rc;
     fs.u64_scratch = (double *)p;
+    if (!fs.zg) {                             /* a batch's discs are advanced by the batch */
+        const char *e = getenv("CYLMHD_STAGE_KERNELS");
+        fs.split = e && e[0] == '1' ? false : (e && e[0] == '2' ? true : fs.fp32);
+    }
+    if (fs.split && (rc = alloc(9 * NS * fs.esize, &fs.div))) return rc;
     const int TJ = Tile<T, 0>::TJ;
     const int TK = Tile<T, 0>::TK;
     fs.tiles_j = (g.NY - 2 + TJ - 1) / TJ;
@@ -292,7 +338,7 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
      * segments of at least 8 planes. */
     int ncu = 256;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || ncu < 1) ncu = 256;
-    const double slots = 2.0 * ncu;
+    const double slots = (fs.split && fs.fp32 ? 3.0 : 2.0) * ncu;     /* resident stage blocks */
     const int planes = g.NX - 2;
     int nseg = 1;
     double bestc = 1e300;
@@ -323,6 +369,11 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     }
     fs.tuned = fs.ncand < 2;
     fs.dt_blocks = std::max((cols * maxseg + 7) / 8 * 8, 1024);
+    if (fs.split) {
+        int nch;
+        fs.dt_blocks = std::max(fs.dt_blocks, update_blocks(g, nch));
+    }
+    fs.dt_nb = (cols * fs.nseg + 7) / 8 * 8;
     if ((rc = alloc(2 * (size_t)fs.dt_blocks * sizeof(double), &p))) return rc;
     fs.dtpart = (double *)p;
     const size_t nik = (size_t)g.NX * g.NZ;
@@ -356,6 +407,7 @@ static FArgs<T> make_args(FusedState &fs, const DevGeom &g, const DevState &d, c
     a.Un = (const T *)fs.Un;
     a.shadow = (T *)fs.shadow;
     a.itab = fs.itab; a.ikd = fs.ikd; a.ktab = fs.ktab; a.ikg = (const T *)fs.ikg;
+    a.div = (T *)fs.div;
     return a;
 }
 
@@ -394,9 +446,9 @@ static void setdt_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
         nb = (int)nblocks(M, 256, (unsigned)std::min(fs.dt_blocks, 4096));
         hipLaunchKernelGGL(k_setdt_state<T>, dim3(nb), dim3(256), 0, s, a);
     } else {
-        nb = (fs.tiles_j * fs.tiles_k * fs.nseg + 7) / 8 * 8;
+        nb = fs.dt_nb;
     }
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, s, fs.dtpart, nb, d.red);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1024), 0, s, fs.dtpart, nb, d.red);
 }
 
 int setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c)
@@ -423,9 +475,13 @@ static void tune_segments(hipStream_t s, FusedState &fs, FArgs<T> a)
         a.seglen = (planes + ns - 1) / ns;
         a.nblocks = fs.tiles_j * fs.tiles_k * ((planes + a.seglen - 1) / a.seglen);
         const int nb = (a.nblocks + 7) / 8 * 8;
-        hipLaunchKernelGGL((k_fused<T, LIM, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        auto launch = [&]() {
+            if (fs.split) hipLaunchKernelGGL((k_march<T, LIM, HLL>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+            else hipLaunchKernelGGL((k_fused<T, LIM, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        };
+        launch();
         hipEventRecord(e0, s);
-        for (int r = 0; r < 2; r++) hipLaunchKernelGGL((k_fused<T, LIM, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        for (int r = 0; r < 2; r++) launch();
         hipEventRecord(e1, s);
         float ms = 0.0f;
         if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < best) {
@@ -468,7 +524,16 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
     }
     const int nb = (a.nblocks + 7) / 8 * 8;
     if (ev0) hipEventRecord(ev0, s);
-    hipLaunchKernelGGL((k_fused<T, LIM, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+    if (fs.split) {
+        int nch;
+        const int nu = update_blocks(g, nch);
+        hipLaunchKernelGGL((k_march<T, LIM, HLL>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        hipLaunchKernelGGL((k_update<T, ALT>), dim3(nu), dim3(256), 0, s, a, nch);
+        if (last) fs.dt_nb = nu;
+    } else {
+        hipLaunchKernelGGL((k_fused<T, LIM, HLL, ALT>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        if (last) fs.dt_nb = nb;
+    }
     if (ev1) hipEventRecord(ev1, s);
     const size_t nghost = 2 * (size_t)g.NY * g.NZ + (size_t)(g.NX - 2) * 2 * g.NZ + (size_t)(g.NX - 2) * (g.NY - 2) * 2;
     hipLaunchKernelGGL(k_halo<T>, dim3(nblocks(nghost, 256, 4096)), dim3(256), 0, s, a);

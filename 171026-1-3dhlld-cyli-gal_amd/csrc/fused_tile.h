@@ -188,6 +188,7 @@ template <typename T> struct FArgs {
     size_t NS;                 /* component stride of the state arrays */
     int NZP;                   /* padded k-row length of the state arrays */
     const T *edge;
+    T *div;                    /* split stage: the divergence of each cell [9][NS] */
     int *counts;
     double *dtpart;
     size_t N;
@@ -323,6 +324,136 @@ __device__ __forceinline__ bool gravity_staged(const DevGeom &g, int i, cdp it, 
     gy = 0.0 + spiralfphi;
     gz = ikd[2] + spiralfz;
     return true;
+}
+
+/* The update of one interior cell (i, gj, gk) once its divergence acc is
+ * formed: SpiralGravityCooling (or the source ALT selects), the RK stage
+ * combination, the shadow values the ghost pass reads, the absorbing
+ * layers (last stage), UtoV and Check, the store and, on the last stage,
+ * the cell's Setdt candidate.  v holds the stage-input primitives of the
+ * cell (overwritten), uin / unn its stage input and U^n (unn zero where the
+ * stage kind reads no U^n), rfl its density floor; the table rows of the
+ * cell come in as ikd (1/minlength or minlength, Grx, Grz, q of (i, k)),
+ * kt (szs, z, 1 + a, z^2 s^2 term, z absorption of k), sc (sin, cos of
+ * the arm phase at (i, gj)), cabs (central absorption of (i, k)) and ub
+ * (its offsets, FP32 state).  cc / ccr: the cell in the state and in the
+ * reference layout; it: the i row of itab. */
+template <typename T, bool ALT, typename TT>
+__device__ __forceinline__ void cell_update(const FArgs<T> &a, int i, int gj, int gk, size_t cc, size_t ccr, cdp it,
+                                            const T *acc, T *v, const T *uin, const T *unn, T rfl, const TT *ikd,
+                                            const TT *kt, const double *sc, const TT *cabs, const T *ub, int &f0,
+                                            int &f1, int &f2, double &best, long long &bix)
+{
+    const DevGeom &g = a.g;
+    const Consts &c = a.c;
+    const size_t N = a.N, NS = a.NS;
+    T S[9], u[9], un[9];
+    double gx, gy, gz;
+    const bool spiral = gravity_staged(g, i, it, ikd, kt, sc, gx, gy, gz, ALT && (c.source == 1 || c.source == 2));
+    sources<T, ALT>(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
+    T ubc[NOFF];                 /* offsets of the cell */
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int q = 0; q < NOFF; q++) ubc[q] = ub[q];
+    }
+#pragma unroll
+    for (int m = 0; m < 9; m++) {
+        /* the empty asm keeps the (zero or loaded) Un value in its own
+         * precision up to here: otherwise the compiler widens it to FP64
+         * at the load, in front of the update barrier, and waits for the
+         * loads there */
+        T w = unn[m];
+        asm("" : "+v"(w));
+        u[m] = uin[m];
+        un[m] = w;
+    }
+    u[8] = T(c.psicoef) * u[8];
+    un[8] = T(c.psicoef) * un[8];             /* Un is zero in stages that do not read it */
+    /* The RK combination is formed in FP64 and rounded once: with FP32
+     * state the stage-2/3 forms would otherwise round four or five times
+     * per update (for FP64 state this is the same arithmetic).  One
+     * branch-free form covers the stage kinds (the coefficients are
+     * uniform), s = ((u + ka Un) - dt div) + dt S, r = kf (kb Un + ke s):
+     * kinds 1, 4: s;  2: 0.25 ((3 Un + U) - dt div + dt S);  3: (1/3) (Un +
+     * 2 s);  5: 0.5 ((Un + U) - dt div + dt S). */
+    const int kind = a.kind;
+    const double dt = c.dt;
+    const double ka = kind == 2 ? 3.0 : (kind == 5 ? 1.0 : 0.0);
+    const double kb = kind == 3 ? 1.0 : 0.0, ke = kind == 3 ? 2.0 : 1.0;
+    const double kf = kind == 2 ? 0.25 : (kind == 3 ? 1.0 / 3.0 : (kind == 5 ? 0.5 : 1.0));
+#pragma unroll
+    for (int m = 0; m < 9; m++) {
+        const double div = acc[m], um = u[m], unm = un[m], sm = S[m];
+        const double sv = ((um + ka * unm) - dt * div) + dt * sm;
+        u[m] = T(kf * (kb * unm + ke * sv));
+    }
+    if (!a.rk3 && gk == 1) u[3] = T(0.0);       /* ZLeftVzAntisymBSym, U3[k=1] */
+    if (i == 1 || i == g.NX - 2 || gj == 1 || gj == g.NY - 2) {
+#pragma unroll
+        for (int m = 0; m < 9; m++) a.shadow[m * NS + cc] = u[m];
+    }
+    if (gk <= 2 || gk == g.NZ - 2) {
+        /* z sides: packed planes k = 1, 2, NZ-2 after the shadow, the nine
+         * values of a cell together */
+        const size_t P = (size_t)g.NX * g.NY;
+        T *zs = a.shadow + 9 * NS + 9 * ((size_t)i * g.NY + gj);
+        if (gk == 1) {
+#pragma unroll
+            for (int m = 0; m < 9; m++) zs[m] = u[m];
+        }
+        if (gk == 2) {
+#pragma unroll
+            for (int m = 0; m < 9; m++) zs[9 * P + m] = u[m];
+        }
+        if (gk == g.NZ - 2) {
+#pragma unroll
+            for (int m = 0; m < 9; m++) zs[18 * P + m] = u[m];
+        }
+    }
+    if (a.last) {
+        const bool ax = i >= g.NX - 11, ac = i <= g.irin, az = !a.rk3 && gk >= g.NZ - 11;
+        if (ax || ac || az) {
+            double ui[9];                 /* every load issued before the first use */
+#pragma unroll
+            for (int m = 0; m < 9; m++) ui[m] = a.Uinit[m * N + ccr];
+            if constexpr (sizeof(T) == 4) {
+#pragma unroll
+                for (int m = 0; m < 9; m++)
+                    if (oslot(m) >= 0) ui[m] -= (double)ubc[oslot(m)];   /* the target as a deviation */
+            }
+#pragma unroll
+            for (int m = 0; m < 9; m++) {
+                if (ax) u[m] -= (T(it[IT_XABS]) * (u[m] - T(ui[m])));
+                if (ac) u[m] -= (T(*cabs) * (u[m] - T(ui[m])));
+                if (az) u[m] -= (T(kt[4]) * (u[m] - T(ui[m])));
+            }
+        }
+    }
+    Prec<T>::utov(c, u, ubc, it[IT_OX], v);
+    const int fl = Prec<T>::check(c, u, ubc, v, it[IT_X], rfl);
+    f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
+#pragma unroll
+    for (int m = 0; m < 9; m++) a.Uout[m * NS + cc] = u[m];
+    if (a.want_dt && i < g.NX - 11) {
+        /* Setdt (setdt.h:33-69) on the checked primitives, FP64 */
+        const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
+        const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
+        const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
+        double t1, tc;
+        if constexpr (sizeof(T) == 4) {
+            /* FP64 still, with the row's stored 1/minlength and the cooling
+             * term only when a cooling function is on (otherwise it is
+             * 0 / (1.5 p) = 0) */
+            t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) * ikd[0];
+            tc = c.cooling ? (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7))) : 0.0;
+        } else {
+            t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / ikd[0];
+            tc = (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7)));
+        }
+        const double tt = rmax(t1, tc);
+        const long long ix = (long long)ccr;
+        if ((tt > best || (tt == best && ix < bix)) && tt > 0.0) { best = tt; bix = ix; }
+    }
 }
 
 /* Tile shapes: a block owns TJ x TK (phi x z) cells and has (TJ+2) x TK
@@ -727,115 +858,10 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
 #pragma unroll
                 for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz[1][m * NZT + qp] - g5 * sz[1][m * NZT + qm]);
             }
-            T v[9], S[9], u[9], un[9];
+            T v[9];
             ld9(s0 + cbase, v);
-            double gx, gy, gz;
-            const bool spiral = gravity_staged(g, i, it, s_ikd[sl][ck], s_kt[ck], s_sc[sl][cj], gx, gy, gz,
-                                               ALT && (c.source == 1 || c.source == 2));
-            sources<T, ALT>(c, v, T(it[IT_X]), T(it[IT_OX]), T(gx), T(gy), T(gz), spiral, S);
-            T ubc[NOFF];                 /* offsets of the cell, staged with the plane's rows */
-            if constexpr (sizeof(T) == 4) {
-#pragma unroll
-                for (int q = 0; q < NOFF; q++) ubc[q] = s_ikg[sl][ck][4 + q];
-            }
-#pragma unroll
-            for (int m = 0; m < 9; m++) {
-                /* the empty asm keeps the (zero or loaded) Un value in its
-                 * own precision up to here: otherwise the compiler widens it
-                 * to FP64 at the load, in front of the update barrier, and
-                 * waits for the loads there */
-                T w = unn[m];
-                asm("" : "+v"(w));
-                u[m] = uin[m];
-                un[m] = w;
-            }
-            u[8] = T(c.psicoef) * u[8];
-            un[8] = T(c.psicoef) * un[8];             /* Un is zero in stages that do not read it */
-            /* The RK combination is formed in FP64 and rounded once: with FP32
-             * state the stage-2/3 forms would otherwise round four or five
-             * times per update (for FP64 state this is the same arithmetic).
-             * One branch-free form covers the stage kinds (the coefficients
-             * are uniform), s = ((u + ka Un) - dt div) + dt S, r = kf (kb Un +
-             * ke s):  kinds 1, 4: s;  2: 0.25 ((3 Un + U) - dt div + dt S);
-             * 3: (1/3) (Un + 2 s);  5: 0.5 ((Un + U) - dt div + dt S). */
-            const int kind = a.kind;
-            const double dt = c.dt;
-            const double ka = kind == 2 ? 3.0 : (kind == 5 ? 1.0 : 0.0);
-            const double kb = kind == 3 ? 1.0 : 0.0, ke = kind == 3 ? 2.0 : 1.0;
-            const double kf = kind == 2 ? 0.25 : (kind == 3 ? 1.0 / 3.0 : (kind == 5 ? 0.5 : 1.0));
-#pragma unroll
-            for (int m = 0; m < 9; m++) {
-                const double div = acc[m], um = u[m], unm = un[m], sm = S[m];
-                const double sv = ((um + ka * unm) - dt * div) + dt * sm;
-                u[m] = T(kf * (kb * unm + ke * sv));
-            }
-            if (!a.rk3 && gk == 1) u[3] = T(0.0);       /* ZLeftVzAntisymBSym, U3[k=1] */
-            if (i == 1 || i == g.NX - 2 || gj == 1 || gj == g.NY - 2) {
-#pragma unroll
-                for (int m = 0; m < 9; m++) a.shadow[m * NS + cc] = u[m];
-            }
-            if (gk <= 2 || gk == g.NZ - 2) {
-                /* z sides: packed planes k = 1, 2, NZ-2 after the shadow,
-                 * the nine values of a cell together */
-                const size_t P = (size_t)g.NX * g.NY;
-                T *zs = a.shadow + 9 * NS + 9 * ((size_t)i * g.NY + gj);
-                if (gk == 1) {
-#pragma unroll
-                    for (int m = 0; m < 9; m++) zs[m] = u[m];
-                }
-                if (gk == 2) {
-#pragma unroll
-                    for (int m = 0; m < 9; m++) zs[9 * P + m] = u[m];
-                }
-                if (gk == g.NZ - 2) {
-#pragma unroll
-                    for (int m = 0; m < 9; m++) zs[18 * P + m] = u[m];
-                }
-            }
-            if (a.last) {
-                const bool ax = i >= g.NX - 11, ac = i <= g.irin, az = !a.rk3 && gk >= g.NZ - 11;
-                if (ax || ac || az) {
-                    double ui[9];                 /* every load issued before the first use */
-#pragma unroll
-                    for (int m = 0; m < 9; m++) ui[m] = a.Uinit[m * N + ccr];
-                    if constexpr (sizeof(T) == 4) {
-#pragma unroll
-                        for (int m = 0; m < 9; m++)
-                            if (oslot(m) >= 0) ui[m] -= (double)ubc[oslot(m)];   /* the target as a deviation */
-                    }
-#pragma unroll
-                    for (int m = 0; m < 9; m++) {
-                        if (ax) u[m] -= (T(it[IT_XABS]) * (u[m] - T(ui[m])));
-                        if (ac) u[m] -= (T(s_cabs[sl][ck]) * (u[m] - T(ui[m])));
-                        if (az) u[m] -= (T(s_kt[ck][4]) * (u[m] - T(ui[m])));
-                    }
-                }
-            }
-            Prec<T>::utov(c, u, ubc, it[IT_OX], v);
-            const int fl = Prec<T>::check(c, u, ubc, v, it[IT_X], rfl);
-            f0 += fl & 1; f1 += (fl >> 1) & 1; f2 += (fl >> 2) & 1;
-#pragma unroll
-            for (int m = 0; m < 9; m++) a.Uout[m * NS + cc] = u[m];
-            if (a.want_dt && i < g.NX - 11) {
-                /* Setdt (setdt.h:33-69) on the checked primitives, FP64 */
-                const double V0 = v[0], V1 = v[1], V2 = v[2], V3 = v[3], V4 = v[4], V5 = v[5], V6 = v[6], V7 = v[7];
-                const double v2 = V1 * V1 + V2 * V2 + V3 * V3;
-                const double b2 = V4 * V4 + V5 * V5 + V6 * V6;
-                double t1, tc;
-                if constexpr (sizeof(T) == 4) {
-                    /* FP64 still, with the row's stored 1/minlength and the
-                     * cooling term only when a cooling function is on
-                     * (otherwise it is 0 / (1.5 p) = 0) */
-                    t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) * s_ikd[sl][ck][0];
-                    tc = c.cooling ? (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7))) : 0.0;
-                } else {
-                    t1 = sqrt(v2 + (c.gm * V7 + b2) / V0) / s_ikd[sl][ck][0];
-                    tc = (2.0 * ((V0 * V0 * cooling(c, V0, V7)) / (1.5 * V7)));
-                }
-                const double tt = rmax(t1, tc);
-                const long long ix = (long long)ccr;
-                if ((tt > best || (tt == best && ix < bix)) && tt > 0.0) { best = tt; bix = ix; }
-            }
+            cell_update<T, ALT>(a, i, gj, gk, cc, ccr, it, acc, v, uin, unn, rfl, s_ikd[sl][ck], s_kt[ck], s_sc[sl][cj],
+                                &s_cabs[sl][ck], &s_ikg[sl][ck][4], f0, f1, f2, best, bix);
         } else if (is_cell) {
 #pragma unroll
             for (int m = 0; m < 9; m++) fx_prev[m] = fxn[m];
@@ -859,6 +885,358 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
         long long *sx = (long long *)(sv + NTH / 64);
         block_dt_max<NTH / 64>(best, bix, sv, sx);
         if (t == 0) { a.dtpart[2 * slot] = best; a.dtpart[2 * slot + 1] = (double)bix; }
+    }
+}
+
+/* ---- the stage in two kernels ----------------------------------------------
+ * march_tile: the r-march of fused_tile without the update: at plane i the
+ * phi- and z-states and faces, the r-states of cell i+1 and the r-face
+ * flux at i+1/2, and the divergence of each cell of plane i, stored to
+ * a.div; update_cell then does the rest of the stage per cell, streaming
+ * (cell_update, the same arithmetic as fused_tile).  Without the update's
+ * registers and tables, and with one padded plane in LDS instead of three
+ * (each thread keeps the primitives of its own cell at planes i+1 and i+2
+ * in registers: only the r-states read them, and the cell at i+1 is
+ * written into the plane buffer once plane i is done), the march needs
+ * 51 KB of LDS and few enough registers for three 256-thread blocks per CU.
+ * Results are those of fused_tile bit for bit. */
+template <typename T, int LIM, bool HLL, int SHAPE>
+__device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
+{
+    constexpr int TJ = Tile<T, SHAPE>::TJ, TK = Tile<T, SHAPE>::TK;
+    constexpr int NTH = (TJ + 2) * TK;
+    constexpr int NZT = TJ * (TK + 2);
+    constexpr int NCELL = TJ * TK, NHALO = 4 * TK + 4 * TJ;
+    static_assert(NZT <= NTH && TJ * (TK + 1) <= NTH && NHALO <= NTH, "tile does not fit the block");
+    static_assert(TK <= NTH - NCELL && TK + 2 <= NTH, "no spare threads for the table rows");
+    constexpr bool MC = LIM == 1;
+    constexpr int PK = TK + 4, PS = (TJ + 4) * PK, SLOT = 9 * PS;
+    const DevGeom &g = a.g;
+    const Consts &c = a.c;
+    /* plane i, padded as a ring slot of fused_tile */
+    __shared__ T plane[SLOT];
+    __shared__ __align__(16) T sy[2][9 * NTH];
+    __shared__ T sz[2][9 * NZT];
+    __shared__ T s_g[2][TK][4];                  /* g1, g2, g3, g5 of plane i (slot i % 2) */
+    __shared__ T s_odz[TK + 2][2];
+    __shared__ T s_vz[MC ? 2 : 1][MC ? TJ : 1][9];
+    static_assert(2 * TJ * 9 <= NTH, "no threads for the z-boundary rows");
+
+    const int t = threadIdx.x;
+    const int tj = tile / a.tiles_k, tk = tile % a.tiles_k;
+    const int j0 = 1 + tj * TJ, k0 = 1 + tk * TK;
+    const int kend = min(k0 + TK - 1, g.NZ - 2);
+    const bool zedge = k0 == 1 || kend == g.NZ - 2;
+    const int ibeg = 1 + seg * a.seglen;
+    const int iend = min(g.NX - 2, ibeg + a.seglen - 1);
+    if (ibeg > iend) return;
+    if (t < TK + 2) {
+        const int zg = kmap(g, k0 + t - 1);
+        s_odz[t][0] = T(ro(a.ktab, (size_t)zg * KT + KT_ODZ));
+        s_odz[t][1] = T(ro(a.ktab, (size_t)(zg > 0 ? zg - 1 : 0) * KT + KT_ODZ));
+    }
+    const bool is_cell = t < NCELL;
+    const int cj = t / TK, ck = t % TK;
+    const int gj = j0 + cj, gk = k0 + ck;
+    const bool valid = is_cell && gj <= g.NY - 2 && gk <= g.NZ - 2;
+    const size_t NS = a.NS;
+    const int cbase = (cj + 2) * PK + ck + 2;
+    const int yr = t / TK - 1, yc = t % TK;
+    const int ybase = (yr + 1) * PK + yc + 2;
+    const bool zstate = t < NZT;
+    const int zr = t / (TK + 2), zc = t % (TK + 2) - 1;
+    const int zbase = (zr + 2) * PK + zc + 1;
+    const bool yface = t < (TJ + 1) * TK;
+    const int fyj = t / TK, fyk = t % TK;
+    const bool zface = t < TJ * (TK + 1);
+    const int fzj = t / (TK + 1), fzk = t % (TK + 1);
+    const int kf = k0 + fzk - 1;
+    const int cjm = jmap(g, gj), ckm = kmap(g, gk);       /* own column, clamped in partial tiles */
+
+    auto ld9 = [&](int off, T *v) {
+#pragma unroll
+        for (int m = 0; m < 9; m++) v[m] = plane[off + m * PS];
+    };
+    auto st9 = [&](int off, const T *v) {
+#pragma unroll
+        for (int m = 0; m < 9; m++) plane[off + m * PS] = v[m];
+    };
+    auto col_V = [&](int p, T *v) { cell_V(a, SI(p, cjm, ckm), p, ckm, v); };
+    int hj = 0, hk = 0;
+    if (t < 2 * TK) { hj = -2 + t / TK; hk = t % TK; }
+    else if (t < 4 * TK) { hj = TJ + (t - 2 * TK) / TK; hk = (t - 2 * TK) % TK; }
+    else if (t < 4 * TK + 2 * TJ) { const int u = t - 4 * TK; hj = u / 2; hk = -2 + u % 2; }
+    else { const int u = t - 4 * TK - 2 * TJ; hj = u / 2; hk = TK + u % 2; }
+    const size_t hcol = (size_t)jmap(g, j0 + hj) * a.NZP + kmap(g, k0 + hk) + KOFF;
+    const int hoff = (hj + 2) * PK + hk + 2;
+
+    T up_prev[9], fx_prev[9], vn[9];                 /* vn: own cell at plane i+1 */
+    const size_t NE = (size_t)g.NY * g.NZ;
+    /* ---- prologue: upper r-state of cell ibeg, the flux at ibeg - 1/2,
+     * plane ibeg's own cell into the plane buffer ---------------------------- */
+    if (is_cell) {
+        if (ibeg == 1) {
+            T v0[9], v1[9];
+            col_V(0, v0); col_V(1, v1); col_V(2, vn);
+            cell_states<0, LIM>(c, v0, v1, vn, T(ro(a.itab, 1 * IT + IT_ODX)), T(ro(a.itab, 0 * IT + IT_ODX)), T(1.0), up_prev, (T *)nullptr);
+            const size_t e = (size_t)cjm * g.NZ + ckm;
+#pragma unroll
+            for (int m = 0; m < 9; m++) fx_prev[m] = a.edge[m * NE + e];   /* face 0 */
+            st9(cbase, v1);
+        } else {
+            T vm2[9], vm1[9], v0[9], upm[9], lo[9];
+            col_V(ibeg - 2, vm2); col_V(ibeg - 1, vm1); col_V(ibeg, v0); col_V(ibeg + 1, vn);
+            cell_states<0, LIM>(c, vm2, vm1, v0, T(ro(a.itab, (ibeg - 1) * IT + IT_ODX)), T(ro(a.itab, (ibeg - 2) * IT + IT_ODX)), T(1.0), upm, (T *)nullptr);
+            cell_states<0, LIM>(c, vm1, v0, vn, T(ro(a.itab, ibeg * IT + IT_ODX)), T(ro(a.itab, (ibeg - 1) * IT + IT_ODX)), T(1.0), up_prev, lo);
+            riemann<0, HLL>(c, upm, lo, T(ro(a.itab, (ibeg - 1) * IT + IT_XM)), T(ro(a.itab, (ibeg - 1) * IT + IT_OX)), fx_prev);
+            st9(cbase, v0);
+        }
+    }
+
+    /* Order of the plane buffer: the own cell of plane i+1 is written after
+     * the third barrier of iteration i (every read of plane i is before
+     * it), the halo of plane i+1 at the top of iteration i+1 (halo cells
+     * are read only by the state pass, before the second barrier). */
+    for (int i = ibeg; i <= iend; i++) {
+        /* own cell of plane i+2: loads issued here, converted after the
+         * face passes */
+        const bool pre = i + 2 <= g.NX - 1;
+        T raw[9], rub[NOFF];
+        if (pre && is_cell) {
+            const size_t cp = SI(i + 2, cjm, ckm);
+            ub_load(a, i + 2, ckm, rub);
+#pragma unroll
+            for (int m = 0; m < 9; m++) raw[m] = a.Uin[m * NS + cp];
+        }
+        if (t < NHALO) {
+            T v[9];
+            cell_V(a, (size_t)i * g.NY * a.NZP + hcol, i, kmap(g, k0 + hk), v);
+            st9(hoff, v);
+        }
+        {
+            const int u = t - NCELL;
+            if (u >= 0 && u < TK) {
+                const size_t ik = (size_t)i * g.NZ + min(k0 + u, g.NZ - 1);
+#pragma unroll
+                for (int q = 0; q < 4; q++) s_g[i & 1][u][q] = ro(a.ikg, ikg_w<T>() * ik + q);
+            }
+        }
+        if (MC && zedge) {
+            const int side = t / (TJ * 9), q = t % (TJ * 9), rj = q / 9, m = q % 9;
+            if (side < 2 && ((side == 0 && k0 == 1) || (side == 1 && kend == g.NZ - 2))) {
+                const size_t NP = (size_t)g.NX * g.NY;
+                s_vz[side][rj][m] = T(ro(a.Vinit_b, (size_t)m * 2 * NP + side * NP + (size_t)i * g.NY + jmap(g, j0 + rj)));
+            }
+        }
+        lds_barrier();
+        const cdp it = (cdp)a.itab + (size_t)i * IT;
+        /* ---- limiter states, once per cell and direction ---------------- */
+        {
+            T vm[9], v0[9], vp[9], up[9], lo[9];
+            const T f1 = T(it[IT_OX]), f2 = T(g.oneody);
+            ld9(ybase, vm); ld9(ybase + PK, v0); ld9(ybase + 2 * PK, vp);
+            cell_states<1, LIM>(c, vm, v0, vp, f1, f1, f2, up, lo);
+#pragma unroll
+            for (int m = 0; m < 9; m++) { sy[0][m * NTH + t] = up[m]; sy[1][m * NTH + t] = lo[m]; }
+        }
+        if (zstate) {
+            T vm[9], v0[9], vp[9], up[9], lo[9];
+            ld9(zbase, vm); ld9(zbase + 1, v0); ld9(zbase + 2, vp);
+            cell_states<2, LIM>(c, vm, v0, vp, s_odz[zc + 1][0], s_odz[zc + 1][1], T(1.0), up, lo);
+#pragma unroll
+            for (int m = 0; m < 9; m++) { sz[0][m * NZT + t] = up[m]; sz[1][m * NZT + t] = lo[m]; }
+        }
+        lds_barrier();
+        /* ---- phi faces: (TJ+1) x TK ------------------------------------- */
+        if (yface) {
+            T vl[9], vr[9], fy[9];
+            const int qu = fyj * TK + fyk, ql = (fyj + 1) * TK + fyk;
+#pragma unroll
+            for (int m = 0; m < 9; m++) { vl[m] = sy[0][m * NTH + qu]; vr[m] = sy[1][m * NTH + ql]; }
+            if (LIM == 2) {
+                const int jf = j0 + fyj - 1;
+                const size_t NQ = (size_t)g.NX * g.NZ, ik = (size_t)i * g.NZ + min(k0 + fyk, g.NZ - 1);
+                if (jf == 0) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vl[m] = T(ro(a.Vinit_b, (size_t)m * 2 * NQ + ik));
+                } else if (jf == g.NY - 2) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vr[m] = T(ro(a.Vinit_b, (size_t)m * 2 * NQ + NQ + ik));
+                }
+            }
+            riemann<1, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fy);
+#pragma unroll
+            for (int m = 0; m < 9; m++) sy[1][m * NTH + ql] = fy[m];
+        }
+        /* ---- z faces: TJ x (TK+1) --------------------------------------- */
+        if (zface && kf <= g.NZ - 2) {
+            T vl[9], vr[9], fz[9];
+            const int qu = fzj * (TK + 2) + fzk, ql = qu + 1;
+            if (kf == 0) {
+                if (MC) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vl[m] = s_vz[0][fzj][m];
+                } else if (LIM == 2) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vl[m] = sz[1][m * NZT + ql];
+                } else if (LIM == 3) {
+                    T vm[9], v0[9], vp[9];
+                    const int jj = jmap(g, j0 + fzj), L = g.NZ - 2;
+                    cell_V(a, SI(i, jj, L - 1), i, L - 1, vm); cell_V(a, SI(i, jj, L), i, L, v0);
+                    cell_V(a, SI(i, jj, L + 1), i, L + 1, vp);
+                    cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), vl, (T *)nullptr);
+                } else {
+                    T vm[9], v0[9], vp[9];
+                    const int zb = (fzj + 2) * PK + 2;
+                    ld9(zb, vm); ld9(zb + 1, v0); ld9(zb + 2, vp);
+                    cell_states<2, LIM>(c, vm, v0, vp, T(ro(a.ktab, 2 * KT + KT_ODZ)), T(ro(a.ktab, 1 * KT + KT_ODZ)), T(1.0), (T *)nullptr, vl);
+                    vl[3] = -vl[3];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 9; m++) vl[m] = sz[0][m * NZT + qu];
+            }
+            if (kf == g.NZ - 2) {
+                if (MC) {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vr[m] = s_vz[1][fzj][m];
+                } else if (LIM == 3) {
+                    T vm[9], v0[9], vp[9];
+                    const int jj = jmap(g, j0 + fzj);
+                    cell_V(a, SI(i, jj, 0), i, 0, vm);
+                    cell_V(a, SI(i, jj, 1), i, 1, v0); cell_V(a, SI(i, jj, 2), i, 2, vp);
+                    cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), (T *)nullptr, vr);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 9; m++) vr[m] = vl[m];
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 9; m++) vr[m] = sz[1][m * NZT + ql];
+            }
+            riemann<2, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fz);
+#pragma unroll
+            for (int m = 0; m < 9; m++) sz[1][m * NZT + ql] = fz[m];
+        }
+        /* ---- x: r-states of cell i+1 (planes i, i+1, i+2 of the own
+         * column) and the flux at i+1/2 ----------------------------------- */
+        T fxn[9], vq[9];
+        if (is_cell) {
+            if (i == g.NX - 2) {
+                const size_t e = (size_t)cjm * g.NZ + ckm;
+#pragma unroll
+                for (int m = 0; m < 9; m++) fxn[m] = a.edge[9 * NE + m * NE + e];   /* face NX-2 */
+            } else {
+                T vm[9], lo[9], up[9];
+                raw[8] = T(c.psicoef) * raw[8];
+                Prec<T>::utov(c, raw, rub, ro(a.itab, (size_t)(i + 2) * IT + IT_OX), vq);
+                ld9(cbase, vm);
+                cell_states<0, LIM>(c, vm, vn, vq, T(ro(a.itab, (i + 1) * IT + IT_ODX)), T(it[IT_ODX]), T(1.0), up, lo);
+                riemann<0, HLL>(c, up_prev, lo, T(it[IT_XM]), T(it[IT_OX]), fxn);
+#pragma unroll
+                for (int m = 0; m < 9; m++) up_prev[m] = up[m];
+            }
+        }
+        lds_barrier();
+        /* ---- divergence, in the reference's order ----------------------- */
+        if (valid) {
+            const int sl = i & 1;
+            T acc[9];
+            {
+                const T g1 = s_g[sl][ck][0], g2 = s_g[sl][ck][1];
+#pragma unroll
+                for (int m = 0; m < 9; m++) { acc[m] = g2 * fxn[m] - g1 * fx_prev[m]; fx_prev[m] = fxn[m]; }
+            }
+            {
+                const T g3 = s_g[sl][ck][2], g4 = g3;
+                const int qm = (cj + 1) * TK + ck, qp = qm + TK;
+#pragma unroll
+                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g4 * sy[1][m * NTH + qp] - g3 * sy[1][m * NTH + qm]);
+            }
+            {
+                const T g5 = s_g[sl][ck][3], g6 = g5;
+                const int qm = cj * (TK + 2) + ck + 1, qp = qm + 1;
+#pragma unroll
+                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz[1][m * NZT + qp] - g5 * sz[1][m * NZT + qm]);
+            }
+            const size_t cc = SI(i, gj, gk);
+#pragma unroll
+            for (int m = 0; m < 9; m++) a.div[m * NS + cc] = acc[m];
+        } else if (is_cell) {
+#pragma unroll
+            for (int m = 0; m < 9; m++) fx_prev[m] = fxn[m];
+        }
+        /* own cell of plane i+1 into the plane buffer; i+2 becomes i+1 */
+        if (is_cell && i < iend) {
+            st9(cbase, vn);
+#pragma unroll
+            for (int m = 0; m < 9; m++) vn[m] = vq[m];
+        }
+    }
+}
+
+/* The per-cell half of the split stage: one thread per interior cell, a
+ * wavefront over 64 consecutive k of one (i, j) row (chunk `ch` of the
+ * row's nch = ceil((NZ-2)/64)), four wavefronts to a block; the stage
+ * input is read once (UtoV gives the primitives the sources use, as the
+ * plane loads of the march do), the divergence from a.div.  Floor counters
+ * and the Setdt partial (slot blockIdx.x) as in fused_tile. */
+template <typename T, bool ALT>
+__device__ __forceinline__ void update_cell(const FArgs<T> &a, int nch)
+{
+    const DevGeom &g = a.g;
+    const size_t NS = a.NS;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int nj = g.NY - 2, nrows = (g.NX - 2) * nj;
+    const int r = w / nch, k = 1 + (w - r * nch) * 64 + lane;
+    int f0 = 0, f1 = 0, f2 = 0;
+    double best = 0.0;
+    long long bix = LLONG_MAX;
+    if (r < nrows && k <= g.NZ - 2) {
+        const int i = 1 + r / nj, j = 1 + r % nj;
+        const cdp it = (cdp)a.itab + (size_t)i * IT;
+        const double sc[2] = {ro(g.sincos, (size_t)i * g.NY + j), ro(g.sincos, (size_t)g.NX * g.NY + (size_t)i * g.NY + j)};
+        const size_t cc = SI(i, j, k), ccr = C3(i, j, k), ik = (size_t)i * g.NZ + k;
+        T uin[9], unn[9], acc[9], v[9], ub[NOFF];
+#pragma unroll
+        for (int m = 0; m < 9; m++) { uin[m] = a.Uin[m * NS + cc]; unn[m] = T(0); }
+        if (a.kind == 2 || a.kind == 3 || a.kind == 5) {
+#pragma unroll
+            for (int m = 0; m < 9; m++) unn[m] = a.Un[m * NS + cc];
+        }
+#pragma unroll
+        for (int m = 0; m < 9; m++) acc[m] = a.div[m * NS + cc];
+        const T rfl = a.rhofloor[cc];
+        ub_load(a, i, k, ub);
+        double ikd[4], kt[5], cabs = 0.0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) ikd[q] = a.ikd[4 * ik + q];
+        const double *kr = a.ktab + (size_t)k * KT;
+        kt[0] = kr[KT_SZS]; kt[1] = kr[KT_Z]; kt[2] = kr[KT_ONEPA]; kt[3] = kr[KT_OZ2S2]; kt[4] = kr[KT_ZABS];
+        if (i <= g.irin) cabs = g.cabs[ik];
+        {
+            T u[9];
+#pragma unroll
+            for (int m = 0; m < 9; m++) u[m] = uin[m];
+            u[8] = T(a.c.psicoef) * u[8];
+            Prec<T>::utov(a.c, u, ub, it[IT_OX], v);
+        }
+        cell_update<T, ALT>(a, i, j, k, cc, ccr, it, acc, v, uin, unn, rfl, ikd, kt, sc, &cabs, ub, f0, f1, f2, best, bix);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        f0 += __shfl_xor(f0, off); f1 += __shfl_xor(f1, off); f2 += __shfl_xor(f2, off);
+    }
+    if (lane == 0) {
+        if (f0) atomicAdd(&a.counts[0], f0);
+        if (f1) atomicAdd(&a.counts[1], f1);
+        if (f2) atomicAdd(&a.counts[2], f2);
+    }
+    if (a.want_dt) {
+        __shared__ double sv[4];
+        __shared__ long long sx[4];
+        block_dt_max<4>(best, bix, sv, sx);
+        if (threadIdx.x == 0) { a.dtpart[2 * blockIdx.x] = best; a.dtpart[2 * blockIdx.x + 1] = (double)bix; }
     }
 }
 

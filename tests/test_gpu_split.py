@@ -1,0 +1,92 @@
+"""GPU: the FP32 stage as two kernels (the r-march writing each cell's
+divergence, then the per-cell update; the default) against the one-kernel
+stage (CYLMHD_STAGE_KERNELS=1 at context creation).  Both run the same
+arithmetic in the same order, so the states, dt, its argmax cell and the
+floor counts must agree bit for bit after every step, for every limiter,
+solver, RK order and source function, on tiles cut short at the phi and z
+edges and on r-marches split into segments.  The parity of the FP32 path
+with the oracle itself is covered by test_gpu_parity and test_gpu_precision.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import cylmhd as cm
+from cases import disc_case, rand_case
+
+pytestmark = pytest.mark.gpu
+
+
+def make_ctx(p, one_kernel):
+    old = os.environ.get("CYLMHD_STAGE_KERNELS")
+    os.environ["CYLMHD_STAGE_KERNELS"] = "1" if one_kernel else "0"
+    try:
+        return cm.Context(p)
+    finally:
+        if old is None:
+            del os.environ["CYLMHD_STAGE_KERNELS"]
+        else:
+            os.environ["CYLMHD_STAGE_KERNELS"] = old
+
+
+def run(p, Ui, Vi, Uc, Vc, nsteps, one_kernel):
+    q = p.copy()
+    q.path = cm.PATH_FUSED32
+    g = make_ctx(q, one_kernel)
+    g.set_initial(Ui, Vi)
+    g.upload(Uc, Vc)
+    hist = []
+    t = 0.0
+    for _ in range(nsteps):
+        dt, ch, ijk = g.setdt()
+        g.gravity(t)
+        g.step(dt, ch)
+        t += dt
+        U, _ = g.download()
+        hist.append(dict(dt=dt, ch=ch, ijk=ijk, counts=np.array(g.floor_counts()), U=U))
+    g.close()
+    return hist
+
+
+def assert_same(p, Ui, Vi, Uc, Vc, nsteps):
+    split = run(p, Ui, Vi, Uc, Vc, nsteps, one_kernel=False)
+    one = run(p, Ui, Vi, Uc, Vc, nsteps, one_kernel=True)
+    for n, (a, b) in enumerate(zip(split, one)):
+        nd = int(np.count_nonzero(a["U"] != b["U"]))
+        print(f"step {n}: dt {a['dt']:.17g} / {b['dt']:.17g}, cells differing {nd}, counts {a['counts'].ravel()}")
+        assert a["dt"] == b["dt"] and a["ch"] == b["ch"] and a["ijk"] == b["ijk"]
+        np.testing.assert_array_equal(a["counts"], b["counts"])
+        np.testing.assert_array_equal(a["U"], b["U"])
+    return split
+
+
+@pytest.mark.parametrize("limiter", [cm.LIMITER_MINMOD, cm.LIMITER_MC, cm.LIMITER_VANLEER, cm.LIMITER_SUPERBEE])
+@pytest.mark.parametrize("solver", [cm.SOLVER_HLLD, cm.SOLVER_HLL])
+def test_split_disc(gpu, limiter, solver):
+    # 30 x 22 x 19 interior cells: partial tiles in phi (14) and z (16)
+    p, Ui, Vi, Uc, Vc = disc_case(2, 30, 22, 19, limiter=limiter, solver=solver)
+    assert_same(p, Ui, Vi, Uc, Vc, 3)
+
+
+@pytest.mark.parametrize("rk", [3, 2])
+@pytest.mark.parametrize("limiter,solver", [(cm.LIMITER_MINMOD, cm.SOLVER_HLLD), (cm.LIMITER_MC, cm.SOLVER_HLLD),
+                                             (cm.LIMITER_VANLEER, cm.SOLVER_HLL), (cm.LIMITER_SUPERBEE, cm.SOLVER_HLLD)])
+def test_split_random_state(gpu, rk, limiter, solver):
+    # every HLLD branch, the cooling band and all three floors
+    p, Ui, Vi, Uc, Vc = rand_case(rk=rk, limiter=limiter, solver=solver, cooling_band=True)
+    hist = assert_same(p, Ui, Vi, Uc, Vc, 3)
+    assert hist[0]["counts"].sum() > 0
+
+
+@pytest.mark.parametrize("source", [1, 2, 3, 4])
+def test_split_sources(gpu, source):
+    p, Ui, Vi, Uc, Vc = disc_case(2, 20, 16, 12, source=source)
+    assert_same(p, Ui, Vi, Uc, Vc, 2)
+
+
+def test_split_segments(gpu):
+    # 200 r-planes: the segment timing on the first stage picks among
+    # several segmentations; every choice must give the same states
+    p, Ui, Vi, Uc, Vc = disc_case(3, 200, 30, 34)
+    assert_same(p, Ui, Vi, Uc, Vc, 2)
