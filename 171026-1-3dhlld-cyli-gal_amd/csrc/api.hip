@@ -54,7 +54,8 @@ struct cylmhd_ctx {
     bool batch_layout = false;      /* a disc of a batch (z-side ghosts packed) */
     /* timing of the per-stage hot kernels */
     bool timing = false;
-    hipEvent_t ev[kEventPairs][2];
+    hipEvent_t ev[kEventPairs][3];  /* start, between the two kernels of a split stage, end */
+    bool ev_mid[kEventPairs] = {};
     int ev_used = 0;
     bool ev_ready = false;
 };
@@ -135,7 +136,8 @@ void destroy_ctx(cylmhd_ctx *c)
     if (!c) return;
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->ev_ready)
-        for (int q = 0; q < kEventPairs; q++) { hipEventDestroy(c->ev[q][0]); hipEventDestroy(c->ev[q][1]); }
+        for (int q = 0; q < kEventPairs; q++)
+            for (int e = 0; e < 3; e++) hipEventDestroy(c->ev[q][e]);
     for (void *p : c->allocs) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
     cyl_grid_free(&c->g);
@@ -449,17 +451,19 @@ int cylmhd_step(cylmhd_ctx *c, double dt, double ch)
     const int nst = rk3 ? 3 : 2;
     HIPCHK(hipMemsetAsync(c->ds.counts, 0, 9 * sizeof(int), c->stream));
     for (int s = 1; s <= nst; s++) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipEvent_t e0 = nullptr, em = nullptr, e1 = nullptr;
         if (c->timing && c->ev_used < kEventPairs) {
             e0 = c->ev[c->ev_used][0];
-            e1 = c->ev[c->ev_used][1];
+            em = c->ev[c->ev_used][1];
+            e1 = c->ev[c->ev_used][2];
+            c->ev_mid[c->ev_used] = c->p.path != CYLMHD_PATH_PLAIN64 && c->fs.split;
             c->ev_used++;
         }
         StageArgs a{s, rk3, c->p.limiter == CYLMHD_LIMITER_MC, c->p.solver == CYLMHD_SOLVER_HLL, c->p.limiter};
         if (c->p.path == CYLMHD_PATH_PLAIN64) {
             plain_stage(c->stream, c->dg, c->ds, k, a, e0, e1);
         } else {
-            int rc = fused_stage(c->stream, c->fs, c->dg, c->ds, k, a, e0, e1);
+            int rc = fused_stage(c->stream, c->fs, c->dg, c->ds, k, a, e0, em, e1);
             if (rc) return rc;
         }
     }
@@ -502,23 +506,25 @@ int cylmhd_sync(cylmhd_ctx *c)
 
 int cylmhd_kernel_timing(cylmhd_ctx *c, int which, int reset, double *avg_ms, int *launches)
 {
-    if (!c || which != 0) return CYLMHD_EINVAL;
+    if (!c || which < 0 || which > 2) return CYLMHD_EINVAL;
     if (!c->ev_ready) {
-        for (int q = 0; q < kEventPairs; q++) {
-            HIPCHK(hipEventCreate(&c->ev[q][0]));
-            HIPCHK(hipEventCreate(&c->ev[q][1]));
-        }
+        for (int q = 0; q < kEventPairs; q++)
+            for (int e = 0; e < 3; e++) HIPCHK(hipEventCreate(&c->ev[q][e]));
         c->ev_ready = true;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     double tot = 0.0;
+    int n = 0;
     for (int q = 0; q < c->ev_used; q++) {
+        if (which != 0 && !c->ev_mid[q]) continue;      /* one-kernel stage: no split to report */
+        const int e0 = which == 2 ? 1 : 0, e1 = which == 1 ? 1 : 2;
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev[q][0], c->ev[q][1]));
+        HIPCHK(hipEventElapsedTime(&ms, c->ev[q][e0], c->ev[q][e1]));
         tot += ms;
+        n++;
     }
-    if (avg_ms) *avg_ms = c->ev_used ? tot / c->ev_used : 0.0;
-    if (launches) *launches = c->ev_used;
+    if (avg_ms) *avg_ms = n ? tot / n : 0.0;
+    if (launches) *launches = n;
     if (reset) c->ev_used = 0;
     c->timing = true;
     return CYLMHD_OK;

@@ -27,9 +27,9 @@ struct FusedState {
     int dt_blocks = 0;
     int dt_nb = 0;             /* partials the last stage wrote */
     /* the stage as two kernels, the r-march (writing each cell's
-     * divergence into div, [9][NS]) and a per-cell update (the default for
-     * FP32 state), or as one kernel (FP64 state); CYLMHD_STAGE_KERNELS=1
-     * or 2 in the environment at context creation selects one or two */
+     * divergence into div, [9][NS]) and a per-cell update (the default), or
+     * as one kernel (CYLMHD_STAGE_KERNELS=1 in the environment at context
+     * creation) */
     bool split = false;
     void *div = nullptr;
     bool dt_valid = false;
@@ -69,7 +69,7 @@ using AllocFn = std::function<int(size_t, void **)>;
                      double *V64);                                                                        \
     int setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c);       \
     int stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,        \
-              const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1);                                        \
+              const StageArgs &a, hipEvent_t ev0, hipEvent_t evm, hipEvent_t ev1);                        \
     int diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d);
 namespace f32 { CYL_FUSED_DECLS }
 namespace f64 { CYL_FUSED_DECLS }
@@ -98,10 +98,12 @@ inline int fused_setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const De
 {
     return fs.fp32 ? f32::setdt(s, fs, g, d, c) : f64::setdt(s, fs, g, d, c);
 }
+/* ev0 / ev1 bracket the stage's hot kernel(s); evm, between the r-march
+ * and the per-cell update of the two-kernel stage, is recorded only there */
 inline int fused_stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
-                       const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1)
+                       const StageArgs &a, hipEvent_t ev0, hipEvent_t evm, hipEvent_t ev1)
 {
-    return fs.fp32 ? f32::stage(s, fs, g, d, c, a, ev0, ev1) : f64::stage(s, fs, g, d, c, a, ev0, ev1);
+    return fs.fp32 ? f32::stage(s, fs, g, d, c, a, ev0, evm, ev1) : f64::stage(s, fs, g, d, c, a, ev0, evm, ev1);
 }
 inline int fused_diag(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d)
 {

@@ -324,7 +324,7 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     fs.u64_scratch = (double *)p;
     if (!fs.zg) {                             /* a batch's discs are advanced by the batch */
         const char *e = getenv("CYLMHD_STAGE_KERNELS");
-        fs.split = e && e[0] == '1' ? false : (e && e[0] == '2' ? true : fs.fp32);
+        fs.split = !(e && e[0] == '1');
     }
     if (fs.split && (rc = alloc(9 * NS * fs.esize, &fs.div))) return rc;
     const int TJ = Tile<T, 0>::TJ;
@@ -333,12 +333,17 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     fs.tiles_k = (g.NZ - 2 + TK - 1) / TK;
     const int cols = fs.tiles_j * fs.tiles_k;
     /* Segments of the r-march: the grid is dealt to the CUs in rounds of
-     * (CUs x 2 resident blocks); pick the count that wastes least in the
-     * last round, charging each segment its prologue (~2.5 planes), with
+     * (CUs x resident blocks per CU); pick the count that wastes least in
+     * the last round, charging each segment its prologue (~2.5 planes), with
      * segments of at least 8 planes. */
     int ncu = 256;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess || ncu < 1) ncu = 256;
-    const double slots = (fs.split && fs.fp32 ? 3.0 : 2.0) * ncu;     /* resident stage blocks */
+    int per_cu = 0;                  /* resident stage blocks per CU (registers and LDS) */
+    if (fs.split)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_march<T, 1, false>, tile_threads<T, 0>(), 0);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fused<T, 1, false, false>, tile_threads<T, 0>(), 0);
+    const double slots = (double)std::max(per_cu, 1) * ncu;
     const int planes = g.NX - 2;
     int nseg = 1;
     double bestc = 1e300;
@@ -499,7 +504,7 @@ static void tune_segments(hipStream_t s, FusedState &fs, FArgs<T> a)
 
 template <typename T, int LIM, bool HLL, bool ALT>
 static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
-                    const StageArgs &sa, hipEvent_t ev0, hipEvent_t ev1)
+                    const StageArgs &sa, hipEvent_t ev0, hipEvent_t evm, hipEvent_t ev1)
 {
     FArgs<T> a = make_args<T>(fs, g, d, c);
     const bool last = sa.rk3 ? sa.stage == 3 : sa.stage == 2;
@@ -528,6 +533,7 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
         int nch;
         const int nu = update_blocks(g, nch);
         hipLaunchKernelGGL((k_march<T, LIM, HLL>), dim3(nb), dim3(tile_threads<T, 0>()), 0, s, a);
+        if (evm) hipEventRecord(evm, s);
         hipLaunchKernelGGL((k_update<T, ALT>), dim3(nu), dim3(256), 0, s, a, nch);
         if (last) fs.dt_nb = nu;
     } else {
@@ -541,12 +547,12 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
 }
 
 int stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
-          const StageArgs &a, hipEvent_t ev0, hipEvent_t ev1)
+          const StageArgs &a, hipEvent_t ev0, hipEvent_t evm, hipEvent_t ev1)
 {
     if (fs.zg) return CYLMHD_EINVAL;   /* a batch's disc: the batch advances it */
     /* one kernel per limiter x solver; the default source (SpiralGravityCooling)
      * runs a kernel compiled for it alone */
-#define CYL_STAGE(L, H, A) stage_t<FT, L, H, A>(s, fs, g, d, c, a, ev0, ev1)
+#define CYL_STAGE(L, H, A) stage_t<FT, L, H, A>(s, fs, g, d, c, a, ev0, evm, ev1)
 #define CYL_STAGE_LIM(H, A)                                  \
     switch (a.limiter) {                                      \
     case 0: CYL_STAGE(0, H, A); break;                        \

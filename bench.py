@@ -11,13 +11,16 @@ FP64 comparison.
 
 value     = interior cells x K / wall seconds of K steps (state resident in
             HBM; host<->device traffic is only the scalars of Setdt).
-roofline  = the fused stage kernel: algorithmic HBM bytes per launch
-            (288 B per cell-update / 3 launches, FP32 state) / its average
-            duration from HIP events on the library's stream; traffic (HBM
-            bytes per launch) and valu_issue_frac (the time its VALU
-            instructions take at one wave64 instruction per 4 cycles on each
-            of the 1024 SIMDs at 2.4 GHz, over its duration) from the
-            rocprofv3 counter summaries under profiles/.
+roofline  = the RK stage, two kernels (the r-march: limiter states, HLLD
+            faces and each cell's flux divergence; then the per-cell update:
+            sources, RK combination, UtoV, Check, store): algorithmic HBM
+            bytes per stage (288 B per cell-update / 3 stages, FP32 state) /
+            the pair's average duration from HIP events on the library's
+            stream, each kernel's share under "kernels"; traffic (HBM bytes
+            per stage) and valu_issue_frac (the time the VALU instructions
+            take at one wave64 instruction per 4 cycles on each of the 1024
+            SIMDs at 2.4 GHz, over the measured time) from the rocprofv3
+            counter summaries under profiles/.
 precision = relative L1 / Linf error per conserved variable of the FP32
             path against the plain FP64 path after 100 steps on the same
             dt sequence, with div B and floor hits (precision.py).
@@ -108,6 +111,8 @@ def run_path(cm, np, path, grid, steps, warmup, config=3, text=False):
         t += dt
     ctx.sync()
     el = time.perf_counter() - t0
+    march_ms, nmarch = ctx.kernel_timing(1)     # the two kernels of a split stage, each alone
+    update_ms, _ = ctx.kernel_timing(2)
     kms, nlaunch = ctx.kernel_timing(0, reset=True)
     diag = ctx.diagnostics()
     counts = ctx.floor_counts()
@@ -122,12 +127,14 @@ def run_path(cm, np, path, grid, steps, warmup, config=3, text=False):
     name = cm.PATH_NAMES[path]
     res = dict(path=name, value=ncell * steps / el, ms_per_step=1e3 * el / steps, steps=steps,
                kernel_ms=kms, kernel_launches=nlaunch, interior_cells=ncell, t=t,
+               march_ms=march_ms if nmarch else None, update_ms=update_ms if nmarch else None,
                mass=diag["mass"], energy=diag["energy"], divb_l1=diag["divb_l1"],
                pressure_floor_hits_last_step=int(counts[:, 1].sum()))
     if text_res is not None:
         res["text_output"] = text_res
+    split = f" (r-march {march_ms:.3f} + update {update_ms:.3f})" if nmarch else ""
     log(f"[bench] {name}: {res['value']:.4g} cell-updates/s, {res['ms_per_step']:.2f} ms/step, "
-        f"stage kernel {kms:.3f} ms x {nlaunch}")
+        f"stage kernels {kms:.3f} ms{split} x {nlaunch}")
     return res
 
 
@@ -183,11 +190,11 @@ def run_large(cm, np, steps):
     prec = precision.compare(cm.params_config(5, *grid), steps, cm.PATH_FUSED64, cm.PATH_FUSED32, log=log)
     keep = ("l1_max", "physical", "divb_l1", "mass_rel", "energy_rel", "floor_hits")
     return dict(grid=list(grid), interior_cells=ncell, steps=steps,
-                fused32={k: r32[k] for k in ("value", "ms_per_step", "kernel_ms")},
-                fused64={k: r64[k] for k in ("value", "ms_per_step", "kernel_ms")},
+                fused32={k: r32[k] for k in ("value", "ms_per_step", "kernel_ms", "march_ms", "update_ms")},
+                fused64={k: r64[k] for k in ("value", "ms_per_step", "kernel_ms", "march_ms", "update_ms")},
                 fp32_over_fp64=r32["value"] / r64["value"],
                 roofline=dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                              frac=achieved / HBM_PEAK, kernel="k_fused<float,MC,HLLD>"),
+                              frac=achieved / HBM_PEAK, kernel="stage: k_march<float,MC,HLLD> + k_update<float>"),
                 precision={k: prec[k] for k in keep if k in prec})
 
 
@@ -229,16 +236,27 @@ def same_grid(cm, np, cpu):
 
 
 def counters(grid):
-    """Per-launch HBM traffic and VALU instructions of the FP32 stage kernel
-    from the rocprofv3 summaries under profiles/ (None where absent)."""
-    out = {}
+    """Per-stage HBM traffic and VALU instructions of the FP32 stage kernels
+    (the r-march and the per-cell update, summed over the two, and each
+    alone under "kernels") from the rocprofv3 summaries under profiles/
+    (None where absent)."""
+    out = {"kernels": {}}
     for key, name in (("traffic", "traffic"), ("valu", "counters")):
         f = os.path.join(ROOT, "profiles", f"{name}_fused32_{grid[0]}x{grid[1]}x{grid[2]}.json")
         if os.path.isfile(f):
             with open(f) as fh:
                 d = json.load(fh)
-            out[key] = d.get("bytes_per_launch") if key == "traffic" else d.get("sq_insts_valu_per_launch")
+            field = "bytes_per_launch" if key == "traffic" else "sq_insts_valu_per_launch"
+            out[key] = d.get(field)
+            for kn, kd in d.get("kernels", {}).items():
+                out["kernels"].setdefault(kn, {})[key] = kd.get(field)
     return out
+
+
+def valu_floor_ms(valu):
+    """VALU issue floor: one wave64 VALU instruction holds a SIMD for 4
+    cycles (FP32 unpacked and FP64 alike); 256 CUs x 4 SIMDs at 2.4 GHz."""
+    return valu * 4.0 / (1024 * 2.4e9) * 1e3 if valu else None
 
 
 def main():
@@ -291,16 +309,23 @@ def main():
     cnt = counters(grid)
     traffic = cnt.get("traffic")
     valu = cnt.get("valu")
-    # VALU issue floor: one wave64 VALU instruction holds a SIMD for 4 cycles
-    # (FP32 unpacked and FP64 alike); 256 CUs x 4 SIMDs at 2.4 GHz
-    valu_floor_ms = valu * 4.0 / (1024 * 2.4e9) * 1e3 if valu else None
+    vfloor = valu_floor_ms(valu)
+    kms = main_res["kernel_ms"]
+    # the stage runs as two kernels: the r-march (states, faces, divergence)
+    # and the per-cell update; the roofline is that of the pair per stage
+    kernels = {}
+    for kn, ms in (("k_march", main_res["march_ms"]), ("k_update", main_res["update_ms"])):
+        kc = cnt["kernels"].get(kn, {})
+        kf = valu_floor_ms(kc.get("valu"))
+        kernels[kn] = dict(ms=ms, traffic=kc.get("traffic"), valu_insts_per_launch=kc.get("valu"),
+                           valu_issue_floor_ms=kf, valu_issue_frac=kf / ms if kf and ms else None,
+                           traffic_GBps=kc["traffic"] / (ms * 1e-3) / 1e9 if kc.get("traffic") and ms else None)
     roofline = dict(bound="hbm", achieved=achieved / 1e9 if achieved else None, peak=HBM_PEAK / 1e9, unit="GB/s",
                     frac=achieved / HBM_PEAK if achieved else None, traffic=traffic,
-                    kernel="k_fused<float,MC,HLLD>", algorithmic_bytes_per_launch=bytes_launch,
-                    kernel_ms=main_res["kernel_ms"],
-                    valu_insts_per_launch=valu, valu_issue_floor_ms=valu_floor_ms,
-                    valu_issue_frac=valu_floor_ms / main_res["kernel_ms"] if valu_floor_ms and main_res["kernel_ms"]
-                    else None)
+                    kernel="stage: k_march<float,MC,HLLD> + k_update<float>",
+                    algorithmic_bytes_per_launch=bytes_launch, kernel_ms=kms,
+                    valu_insts_per_launch=valu, valu_issue_floor_ms=vfloor,
+                    valu_issue_frac=vfloor / kms if vfloor and kms else None, kernels=kernels)
     out = {
         "metric": METRIC,
         "value": main_res["value"],
@@ -322,7 +347,8 @@ def main():
     if "text_output" in main_res:
         out["text_output"] = main_res["text_output"]
     if fp64:
-        out["fp64"] = {k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms", "steps")} for k, v in fp64.items()}
+        out["fp64"] = {k: {kk: v[kk] for kk in ("value", "ms_per_step", "kernel_ms", "march_ms", "update_ms", "steps")}
+                       for k, v in fp64.items()}
         out["fp32_over_fp64"] = {k: main_res["value"] / v["value"] for k, v in fp64.items()}
     if a.precision_steps > 0:
         import precision

@@ -193,7 +193,11 @@ int cylmhd_diagnostics(cylmhd_ctx *c, cylmhd_diag *d);
 int cylmhd_sync(cylmhd_ctx *c);
 /* Average duration (ms) of the named kernel class over the launches since
  * the last reset, timed with HIP events on the context stream:
- * which = 0: the per-stage update kernel(s) of the active path. */
+ * which = 0: the per-stage kernel(s) of the active path (the flux and update
+ * passes of the plain path; the stage kernel, or the r-march and the
+ * per-cell update together, of the fused paths); which = 1 / 2: the r-march
+ * / the per-cell update of a two-kernel stage alone (launches = 0 where the
+ * stage is one kernel). */
 int cylmhd_kernel_timing(cylmhd_ctx *c, int which, int reset, double *avg_ms, int *launches);
 const char *cylmhd_error_string(int code);
 

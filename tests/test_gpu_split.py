@@ -1,11 +1,12 @@
-"""GPU: the FP32 stage as two kernels (the r-march writing each cell's
+"""GPU: the fused stage as two kernels (the r-march writing each cell's
 divergence, then the per-cell update; the default) against the one-kernel
-stage (CYLMHD_STAGE_KERNELS=1 at context creation).  Both run the same
-arithmetic in the same order, so the states, dt, its argmax cell and the
-floor counts must agree bit for bit after every step, for every limiter,
-solver, RK order and source function, on tiles cut short at the phi and z
-edges and on r-marches split into segments.  The parity of the FP32 path
-with the oracle itself is covered by test_gpu_parity and test_gpu_precision.
+stage (CYLMHD_STAGE_KERNELS=1 at context creation), FP32 and FP64 state.
+Both run the same arithmetic in the same order, so the states, dt, its
+argmax cell and the floor counts must agree bit for bit after every step,
+for every limiter, solver, RK order and source function, on tiles cut short
+at the phi and z edges and on r-marches split into segments.  The parity of
+the fused paths with the oracle itself is covered by test_gpu_parity and
+test_gpu_precision.
 """
 import os
 
@@ -30,9 +31,9 @@ def make_ctx(p, one_kernel):
             os.environ["CYLMHD_STAGE_KERNELS"] = old
 
 
-def run(p, Ui, Vi, Uc, Vc, nsteps, one_kernel):
+def run(p, Ui, Vi, Uc, Vc, nsteps, one_kernel, path):
     q = p.copy()
-    q.path = cm.PATH_FUSED32
+    q.path = path
     g = make_ctx(q, one_kernel)
     g.set_initial(Ui, Vi)
     g.upload(Uc, Vc)
@@ -49,9 +50,9 @@ def run(p, Ui, Vi, Uc, Vc, nsteps, one_kernel):
     return hist
 
 
-def assert_same(p, Ui, Vi, Uc, Vc, nsteps):
-    split = run(p, Ui, Vi, Uc, Vc, nsteps, one_kernel=False)
-    one = run(p, Ui, Vi, Uc, Vc, nsteps, one_kernel=True)
+def assert_same(p, Ui, Vi, Uc, Vc, nsteps, path=cm.PATH_FUSED32):
+    split = run(p, Ui, Vi, Uc, Vc, nsteps, False, path)
+    one = run(p, Ui, Vi, Uc, Vc, nsteps, True, path)
     for n, (a, b) in enumerate(zip(split, one)):
         nd = int(np.count_nonzero(a["U"] != b["U"]))
         print(f"step {n}: dt {a['dt']:.17g} / {b['dt']:.17g}, cells differing {nd}, counts {a['counts'].ravel()}")
@@ -63,30 +64,34 @@ def assert_same(p, Ui, Vi, Uc, Vc, nsteps):
 
 @pytest.mark.parametrize("limiter", [cm.LIMITER_MINMOD, cm.LIMITER_MC, cm.LIMITER_VANLEER, cm.LIMITER_SUPERBEE])
 @pytest.mark.parametrize("solver", [cm.SOLVER_HLLD, cm.SOLVER_HLL])
-def test_split_disc(gpu, limiter, solver):
-    # 30 x 22 x 19 interior cells: partial tiles in phi (14) and z (16)
+@pytest.mark.parametrize("path", [cm.PATH_FUSED32, cm.PATH_FUSED64], ids=["fused32", "fused64"])
+def test_split_disc(gpu, limiter, solver, path):
+    # 30 x 22 x 19 interior cells: partial tiles in phi (14 / 6) and z (16)
     p, Ui, Vi, Uc, Vc = disc_case(2, 30, 22, 19, limiter=limiter, solver=solver)
-    assert_same(p, Ui, Vi, Uc, Vc, 3)
+    assert_same(p, Ui, Vi, Uc, Vc, 3, path)
 
 
 @pytest.mark.parametrize("rk", [3, 2])
 @pytest.mark.parametrize("limiter,solver", [(cm.LIMITER_MINMOD, cm.SOLVER_HLLD), (cm.LIMITER_MC, cm.SOLVER_HLLD),
                                              (cm.LIMITER_VANLEER, cm.SOLVER_HLL), (cm.LIMITER_SUPERBEE, cm.SOLVER_HLLD)])
-def test_split_random_state(gpu, rk, limiter, solver):
+@pytest.mark.parametrize("path", [cm.PATH_FUSED32, cm.PATH_FUSED64], ids=["fused32", "fused64"])
+def test_split_random_state(gpu, rk, limiter, solver, path):
     # every HLLD branch, the cooling band and all three floors
     p, Ui, Vi, Uc, Vc = rand_case(rk=rk, limiter=limiter, solver=solver, cooling_band=True)
-    hist = assert_same(p, Ui, Vi, Uc, Vc, 3)
+    hist = assert_same(p, Ui, Vi, Uc, Vc, 3, path)
     assert hist[0]["counts"].sum() > 0
 
 
 @pytest.mark.parametrize("source", [1, 2, 3, 4])
-def test_split_sources(gpu, source):
+@pytest.mark.parametrize("path", [cm.PATH_FUSED32, cm.PATH_FUSED64], ids=["fused32", "fused64"])
+def test_split_sources(gpu, source, path):
     p, Ui, Vi, Uc, Vc = disc_case(2, 20, 16, 12, source=source)
-    assert_same(p, Ui, Vi, Uc, Vc, 2)
+    assert_same(p, Ui, Vi, Uc, Vc, 2, path)
 
 
-def test_split_segments(gpu):
+@pytest.mark.parametrize("path", [cm.PATH_FUSED32, cm.PATH_FUSED64], ids=["fused32", "fused64"])
+def test_split_segments(gpu, path):
     # 200 r-planes: the segment timing on the first stage picks among
     # several segmentations; every choice must give the same states
     p, Ui, Vi, Uc, Vc = disc_case(3, 200, 30, 34)
-    assert_same(p, Ui, Vi, Uc, Vc, 2)
+    assert_same(p, Ui, Vi, Uc, Vc, 2, path)
