@@ -899,6 +899,12 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
  * in registers: only the r-states read them, and the cell at i+1 is
  * written into the plane buffer once plane i is done), the march needs
  * 51 KB of LDS and few enough registers for three 256-thread blocks per CU.
+ * FP64 state (6 x 16 tiles of 128 threads, two waves per SIMD at 256
+ * VGPRs, so four blocks per CU) needs its LDS below 40 KB: there the phi
+ * states, then the phi fluxes and the z states share one buffer -- the phi
+ * faces load their two states into registers, a barrier, then the phi
+ * faces write their fluxes and the z states are written over the phi
+ * states (one barrier more per plane).
  * Results are those of fused_tile bit for bit. */
 template <typename T, int LIM, bool HLL, int SHAPE>
 __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
@@ -915,8 +921,20 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
     const Consts &c = a.c;
     /* plane i, padded as a ring slot of fused_tile */
     __shared__ T plane[SLOT];
-    __shared__ __align__(16) T sy[2][9 * NTH];
-    __shared__ T sz[2][9 * NZT];
+    /* state buffer: phi states [2][9][NTH]; the phi fluxes [9][FYN] (FP64
+     * state: below the z states, in the space of the phi states; FP32:
+     * over the lower phi states) and the z states [2][9][NZT] */
+    constexpr bool SHARE = sizeof(T) == 8;
+    constexpr int FYN = (TJ + 1) * TK;
+    constexpr int SBUF = SHARE ? (2 * 9 * NTH > 9 * FYN + 2 * 9 * NZT ? 2 * 9 * NTH : 9 * FYN + 2 * 9 * NZT)
+                               : 2 * 9 * NTH + 2 * 9 * NZT;
+    __shared__ __align__(16) T sbuf[SBUF];
+    T *const sy0 = sbuf, *const sy1 = sbuf + 9 * NTH;
+    T *const sz0 = SHARE ? sbuf + 9 * FYN : sbuf + 2 * 9 * NTH, *const sz1 = sz0 + 9 * NZT;
+    /* phi flux of face fyj (between rows fyj - 1 and fyj) at column fyk:
+     * pf[m * PFS + (fyj + FOFF) * TK + fyk] */
+    T *const pf = SHARE ? sbuf : sy1;
+    constexpr int PFS = SHARE ? FYN : NTH, FOFF = SHARE ? 0 : 1;
     __shared__ T s_g[2][TK][4];                  /* g1, g2, g3, g5 of plane i (slot i % 2) */
     __shared__ T s_odz[TK + 2][2];
     __shared__ T s_vz[MC ? 2 : 1][MC ? TJ : 1][9];
@@ -1037,22 +1055,28 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
             ld9(ybase, vm); ld9(ybase + PK, v0); ld9(ybase + 2 * PK, vp);
             cell_states<1, LIM>(c, vm, v0, vp, f1, f1, f2, up, lo);
 #pragma unroll
-            for (int m = 0; m < 9; m++) { sy[0][m * NTH + t] = up[m]; sy[1][m * NTH + t] = lo[m]; }
+            for (int m = 0; m < 9; m++) { sy0[m * NTH + t] = up[m]; sy1[m * NTH + t] = lo[m]; }
         }
-        if (zstate) {
-            T vm[9], v0[9], vp[9], up[9], lo[9];
-            ld9(zbase, vm); ld9(zbase + 1, v0); ld9(zbase + 2, vp);
-            cell_states<2, LIM>(c, vm, v0, vp, s_odz[zc + 1][0], s_odz[zc + 1][1], T(1.0), up, lo);
+        auto z_states = [&]() {
+            if (zstate) {
+                T vm[9], v0[9], vp[9], up[9], lo[9];
+                ld9(zbase, vm); ld9(zbase + 1, v0); ld9(zbase + 2, vp);
+                cell_states<2, LIM>(c, vm, v0, vp, s_odz[zc + 1][0], s_odz[zc + 1][1], T(1.0), up, lo);
 #pragma unroll
-            for (int m = 0; m < 9; m++) { sz[0][m * NZT + t] = up[m]; sz[1][m * NZT + t] = lo[m]; }
-        }
+                for (int m = 0; m < 9; m++) { sz0[m * NZT + t] = up[m]; sz1[m * NZT + t] = lo[m]; }
+            }
+        };
+        if (!SHARE) z_states();
         lds_barrier();
         /* ---- phi faces: (TJ+1) x TK ------------------------------------- */
-        if (yface) {
+        {
             T vl[9], vr[9], fy[9];
-            const int qu = fyj * TK + fyk, ql = (fyj + 1) * TK + fyk;
+            if (yface) {
+                const int qu = fyj * TK + fyk, ql = (fyj + 1) * TK + fyk;
 #pragma unroll
-            for (int m = 0; m < 9; m++) { vl[m] = sy[0][m * NTH + qu]; vr[m] = sy[1][m * NTH + ql]; }
+                for (int m = 0; m < 9; m++) { vl[m] = sy0[m * NTH + qu]; vr[m] = sy1[m * NTH + ql]; }
+            }
+            if (yface) {
             if (LIM == 2) {
                 const int jf = j0 + fyj - 1;
                 const size_t NQ = (size_t)g.NX * g.NZ, ik = (size_t)i * g.NZ + min(k0 + fyk, g.NZ - 1);
@@ -1065,8 +1089,18 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                 }
             }
             riemann<1, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fy);
+            }
+            /* every phi state has been read: the phi fluxes and then the z
+             * states may overwrite them */
+            if (SHARE) lds_barrier();
+            if (yface) {
 #pragma unroll
-            for (int m = 0; m < 9; m++) sy[1][m * NTH + ql] = fy[m];
+            for (int m = 0; m < 9; m++) pf[m * PFS + (fyj + FOFF) * TK + fyk] = fy[m];
+            }
+        }
+        if (SHARE) {
+            z_states();
+            lds_barrier();
         }
         /* ---- z faces: TJ x (TK+1) --------------------------------------- */
         if (zface && kf <= g.NZ - 2) {
@@ -1078,7 +1112,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                     for (int m = 0; m < 9; m++) vl[m] = s_vz[0][fzj][m];
                 } else if (LIM == 2) {
 #pragma unroll
-                    for (int m = 0; m < 9; m++) vl[m] = sz[1][m * NZT + ql];
+                    for (int m = 0; m < 9; m++) vl[m] = sz1[m * NZT + ql];
                 } else if (LIM == 3) {
                     T vm[9], v0[9], vp[9];
                     const int jj = jmap(g, j0 + fzj), L = g.NZ - 2;
@@ -1094,7 +1128,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                 }
             } else {
 #pragma unroll
-                for (int m = 0; m < 9; m++) vl[m] = sz[0][m * NZT + qu];
+                for (int m = 0; m < 9; m++) vl[m] = sz0[m * NZT + qu];
             }
             if (kf == g.NZ - 2) {
                 if (MC) {
@@ -1112,11 +1146,11 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                 }
             } else {
 #pragma unroll
-                for (int m = 0; m < 9; m++) vr[m] = sz[1][m * NZT + ql];
+                for (int m = 0; m < 9; m++) vr[m] = sz1[m * NZT + ql];
             }
             riemann<2, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fz);
 #pragma unroll
-            for (int m = 0; m < 9; m++) sz[1][m * NZT + ql] = fz[m];
+            for (int m = 0; m < 9; m++) sz1[m * NZT + ql] = fz[m];
         }
         /* ---- x: r-states of cell i+1 (planes i, i+1, i+2 of the own
          * column) and the flux at i+1/2 ----------------------------------- */
@@ -1149,15 +1183,15 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
             }
             {
                 const T g3 = s_g[sl][ck][2], g4 = g3;
-                const int qm = (cj + 1) * TK + ck, qp = qm + TK;
+                const int qm = (cj + FOFF) * TK + ck, qp = qm + TK;
 #pragma unroll
-                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g4 * sy[1][m * NTH + qp] - g3 * sy[1][m * NTH + qm]);
+                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g4 * pf[m * PFS + qp] - g3 * pf[m * PFS + qm]);
             }
             {
                 const T g5 = s_g[sl][ck][3], g6 = g5;
                 const int qm = cj * (TK + 2) + ck + 1, qp = qm + 1;
 #pragma unroll
-                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz[1][m * NZT + qp] - g5 * sz[1][m * NZT + qm]);
+                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz1[m * NZT + qp] - g5 * sz1[m * NZT + qm]);
             }
             const size_t cc = SI(i, gj, gk);
 #pragma unroll
