@@ -4,21 +4,27 @@
  * Every disc keeps its own context (tables, state, Uinitial, counters), so
  * set-up, dumps and restarts go through the single-disc ABI.  The batch adds:
  *   - a work list of (disc, tile, r-segment) items over all discs, processed
- *     by ONE launch per RK stage whose blocks loop over it (XCD-aware order
- *     within each round of gridDim items);
+ *     by ONE launch per RK stage and tile shape whose blocks loop over it
+ *     (XCD-aware order within each round of gridDim items): either the
+ *     whole stage per item (one kernel), or, as on a single disc, the
+ *     r-march per item (limiter states, faces, each cell's flux divergence)
+ *     followed by ONE per-cell update launch over every disc's cells (two
+ *     kernels); the autotuner picks the form;
  *   - per-disc Setdt on the device: the last stage leaves one partial per
- *     item; k_dt_batch reduces each disc's range, forms dt = cfl/max,
+ *     work item (one kernel) or per update block (two kernels); k_dt_batch
+ *     reduces each disc's range, forms dt = cfl/max,
  *     ch = minminlength * max, psi damping and the spiral amplitude at the
  *     disc's own time, and writes them into that disc's stage arguments;
  *     a disc whose dt falls below dtmin (or is not finite) stops, as main.c
  *     would stop it (main.c:331-334), while the others go on;
  *   - so a whole step needs nothing from the host: it is captured once into
  *     a hipGraph and replayed;
- *   - an autotuner over the tile (its phi x z extents, waves per tile and
- *     register budget), the r-segment length and the grid of the stage
- *     launch, first for all discs together and then per class of disc size
- *     (discs of one phi x z size share a tile; one stage launch per tile in
- *     use), timed with HIP events on the batch stream.
+ *   - an autotuner over the stage form (one or two kernels), the tile (its
+ *     phi x z extents, waves per tile and register budget), the r-segment
+ *     length and the grid of the stage launch, first for all discs together
+ *     and then per class of disc size (discs of one phi x z size share a
+ *     tile; one stage launch per tile in use), timed with HIP events on the
+ *     batch stream.
  */
 #define FUSED_NS b32
 #define FUSED_FT float
@@ -27,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "batch.h"
@@ -42,7 +49,7 @@ struct BDisc {
     const double *logxs;
     double *sincos;
     float *edge;
-    int item_lo, item_hi;
+    int dt_lo, dt_hi;          /* the disc's Setdt partials: its work items or update blocks */
 };
 
 __device__ __forceinline__ int find_disc(const int *__restrict__ off, int n, int b)
@@ -56,7 +63,7 @@ __device__ __forceinline__ int find_disc(const int *__restrict__ off, int n, int
     return lo;
 }
 
-/* the work items [lo, hi) of one tile shape */
+/* the work items [lo, hi) of one tile shape, the whole stage per item */
 template <int LIM, bool HLL, int S, bool ALT>
 __global__ __launch_bounds__((tile_threads<float, S>()), (tile_waves<float, S>())) void k_fused_batch(
     const FArgs<float> *__restrict__ descs, const int4 *__restrict__ items, int lo, int hi, const BDisc *__restrict__ bd)
@@ -67,6 +74,43 @@ __global__ __launch_bounds__((tile_threads<float, S>()), (tile_waves<float, S>()
         if (!bd[w.x].alive) continue;
         fused_tile<float, LIM, HLL, S, ALT, true>(descs[w.x], w.y, w.z, it);
     }
+}
+
+/* the r-march of the work items [lo, hi) of one tile shape (three waves
+ * per SIMD) */
+template <int LIM, bool HLL, int S>
+__global__ __launch_bounds__((tile_threads<float, S>()), 3) void k_march_batch(
+    const FArgs<float> *__restrict__ descs, const int4 *__restrict__ items, int lo, int hi, const BDisc *__restrict__ bd)
+{
+    const int b = remap_block(blockIdx.x, gridDim.x);
+    for (int it = lo + b; it < hi; it += gridDim.x) {
+        const int4 w = items[it];
+        if (!bd[w.x].alive) continue;
+        march_tile<float, LIM, HLL, S, true>(descs[w.x], w.y, w.z);
+        /* the next item's table rows go where this one's last divergence
+         * reads them */
+        lds_barrier();
+    }
+}
+
+/* the per-cell update of every disc, its cells dealt flat (k fastest, 256
+ * to a block, whatever the disc's NZ): disc d's blocks are off[d] ..
+ * off[d+1]-1, block b of the batch writes Setdt partial b */
+template <bool ALT>
+__global__ __launch_bounds__(256) void k_update_batch(const FArgs<float> *__restrict__ descs, const int *__restrict__ off,
+                                                      int n, const BDisc *__restrict__ bd)
+{
+    const int d = find_disc(off, n, blockIdx.x);
+    if (!bd[d].alive) return;
+    const FArgs<float> &a = descs[d];
+    update_cell<float, ALT, true>(a, 0, blockIdx.x - off[d], blockIdx.x);
+}
+
+/* update blocks of one disc: its interior cells, 256 to a block */
+inline int update_blocks_of(int NX, int NY, int NZ)
+{
+    const long long cells = (long long)(NX - 2) * (NY - 2) * (NZ - 2);
+    return (int)((cells + 255) / 256);
 }
 
 /* tile shapes compiled for a limiter x solver: all four for minmod and MC
@@ -95,7 +139,7 @@ __global__ __launch_bounds__(256) void k_halo_batch(const FArgs<float> *__restri
 }
 
 /* Setdt partials of a stored state (before the first step): the blocks of
- * disc d are its work-list slots item_lo..item_hi-1 */
+ * disc d are its partial slots dt_lo..dt_hi-1 */
 __global__ __launch_bounds__(256) void k_setdt_batch(const FArgs<float> *__restrict__ descs, const BDisc *__restrict__ bd,
                                                      const int *__restrict__ off, int n, double *__restrict__ dtpart)
 {
@@ -127,7 +171,7 @@ __global__ __launch_bounds__(256) void k_setdt_batch(const FArgs<float> *__restr
     __shared__ long long sx[4];
     block_dt_max<4>(best, bix, sv, sx);
     if (threadIdx.x == 0) {
-        const int slot = bd[d].item_lo + (int)(blockIdx.x - off[d]);
+        const int slot = bd[d].dt_lo + (int)(blockIdx.x - off[d]);
         dtpart[2 * slot] = best;
         dtpart[2 * slot + 1] = (double)bix;
     }
@@ -145,7 +189,7 @@ __global__ __launch_bounds__(256) void k_dt_batch(FArgs<float> *__restrict__ des
     const int t = threadIdx.x;
     double best = 0.0;
     long long bix = LLONG_MAX;
-    for (int q = D.item_lo + t; q < D.item_hi; q += 256) {
+    for (int q = D.dt_lo + t; q < D.dt_hi; q += 256) {
         const double v = dtpart[2 * q];
         const long long ix = dtpart[2 * q + 1] >= 9.0e18 ? LLONG_MAX : (long long)dtpart[2 * q + 1];
         if ((v > best || (v == best && ix < bix)) && v > 0.0) { best = v; bix = ix; }
@@ -245,10 +289,10 @@ struct cylmhd_batch {
     FArgs<float> *d_descs = nullptr;
     int4 *d_items = nullptr;
     BDisc *d_bd = nullptr;
-    int *d_eoff = nullptr, *d_hoff = nullptr, *d_soff = nullptr, *d_ioff = nullptr;
+    int *d_eoff = nullptr, *d_hoff = nullptr, *d_soff = nullptr, *d_uoff = nullptr, *d_ioff = nullptr;
     double *d_dtpart = nullptr;
-    int nitems = 0, eblocks = 0, hblocks = 0, sblocks = 0;
-    std::vector<int> ioff;
+    int nitems = 0, eblocks = 0, hblocks = 0, sblocks = 0, ublocks = 0;
+    bool split = true;         /* the stage as two kernels (r-march, update) */
     std::vector<void *> allocs;
     float *d_backup = nullptr;
     size_t backup_elems = 0;
@@ -285,7 +329,7 @@ void free_lists(cylmhd_batch *b)
     for (void *p : b->allocs) hipFree(p);
     b->allocs.clear();
     b->d_descs = nullptr; b->d_items = nullptr; b->d_bd = nullptr; b->d_dtpart = nullptr;
-    b->d_eoff = b->d_hoff = b->d_soff = b->d_ioff = nullptr;
+    b->d_eoff = b->d_hoff = b->d_soff = b->d_uoff = b->d_ioff = nullptr;
     b->built = false;
 }
 
@@ -321,8 +365,7 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
     std::vector<int4> items;
     std::vector<FArgs<float>> descs((size_t)nst * n);
     std::vector<BDisc> bd(n);
-    std::vector<int> eoff(n + 1, 0), hoff(n + 1, 0), soff(n + 1, 0);
-    b->ioff.assign(n + 1, 0);
+    std::vector<int> eoff(n + 1, 0), hoff(n + 1, 0), soff(n + 1, 0), uoff(n + 1, 0), ioff(n + 1, 0);
     std::vector<int> ilo(n), ihi(n);
     /* the items of each shape together (discs of a shape in disc order) */
     for (int s = 0; s < kShapes; s++) {
@@ -342,9 +385,11 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
         const DiscView &v = b->view[d];
         int tj, tk, seg, nseg;
         disc_tiling(v, b->dshape[d], b->seglen, tj, tk, seg, nseg);
-        /* block -> disc map of k_setdt_batch: disc d's blocks are
-         * ioff[d] .. ioff[d+1]-1, one per work item of d */
-        b->ioff[d + 1] = b->ioff[d] + (ihi[d] - ilo[d]);
+        /* block -> disc maps: of the update, disc d's blocks are uoff[d] ..
+         * uoff[d+1]-1; of k_setdt_batch, one block per Setdt partial of the
+         * form in use: per update block (uoff) or per work item (ioff) */
+        uoff[d + 1] = uoff[d] + update_blocks_of(v.dg.NX, v.dg.NY, v.dg.NZ);
+        ioff[d + 1] = ioff[d] + (ihi[d] - ilo[d]);
         const Consts c = disc_consts(b->disc[d], 0.0, 0.0);
         for (int s = 0; s < nst; s++) {
             FArgs<float> a{};
@@ -361,6 +406,7 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
             a.nblocks = tj * tk * nseg;
             a.Un = (const float *)v.fs->Un;
             a.shadow = (float *)v.fs->shadow;
+            a.div = (float *)v.fs->div;
             a.itab = v.fs->itab; a.ikd = v.fs->ikd; a.ktab = v.fs->ktab; a.ikg = (const float *)v.fs->ikg;
             const int stage = s + 1;
             const bool last = b->rk3 ? stage == 3 : stage == 2;
@@ -396,11 +442,14 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
         soff[d + 1] = soff[d] + (int)std::min<size_t>(cdiv((size_t)v.dg.NX * v.dg.NY, 256), 256);
     }
     b->nitems = (int)items.size();
-    for (int d = 0; d < n; d++) { bd[d].item_lo = ilo[d]; bd[d].item_hi = ihi[d]; }
-    b->eblocks = eoff[n]; b->hblocks = hoff[n]; b->sblocks = soff[n];
+    for (int d = 0; d < n; d++) {
+        bd[d].dt_lo = b->split ? uoff[d] : ilo[d];
+        bd[d].dt_hi = b->split ? uoff[d + 1] : ihi[d];
+    }
+    b->eblocks = eoff[n]; b->hblocks = hoff[n]; b->sblocks = soff[n]; b->ublocks = uoff[n];
     void *p;
     int rc;
-    if ((rc = balloc(b, &p, 2 * sizeof(double) * (size_t)b->nitems))) return rc;
+    if ((rc = balloc(b, &p, 2 * sizeof(double) * (size_t)std::max(b->ublocks, b->nitems)))) return rc;
     b->d_dtpart = (double *)p;
     for (auto &a : descs) a.dtpart = b->d_dtpart;
     if ((rc = balloc(b, &p, sizeof(FArgs<float>) * descs.size()))) return rc;
@@ -416,6 +465,8 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
     if ((rc = balloc(b, &p, sizeof(int) * (n + 1)))) return rc;
     b->d_soff = (int *)p;
     if ((rc = balloc(b, &p, sizeof(int) * (n + 1)))) return rc;
+    b->d_uoff = (int *)p;
+    if ((rc = balloc(b, &p, sizeof(int) * (n + 1)))) return rc;
     b->d_ioff = (int *)p;
     BCHK(hipMemcpy(b->d_descs, descs.data(), sizeof(FArgs<float>) * descs.size(), hipMemcpyHostToDevice));
     BCHK(hipMemcpy(b->d_items, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice));
@@ -423,7 +474,8 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
     BCHK(hipMemcpy(b->d_eoff, eoff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     BCHK(hipMemcpy(b->d_hoff, hoff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     BCHK(hipMemcpy(b->d_soff, soff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-    BCHK(hipMemcpy(b->d_ioff, b->ioff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(b->d_uoff, uoff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    BCHK(hipMemcpy(b->d_ioff, ioff.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
     b->built = true;
     b->primed = false;
     return CYLMHD_OK;
@@ -436,8 +488,12 @@ void launch_shape(cylmhd_batch *b, const FArgs<float> *descs)
         const int lo = b->groups[S][0], hi = b->groups[S][1];
         if (hi <= lo) return;
         const int G = std::max(8, std::min((hi - lo + 7) / 8 * 8, b->grid));
-        hipLaunchKernelGGL((k_fused_batch<LIM, HLL, S, ALT>), dim3(G), dim3(tile_threads<float, S>()), 0, b->stream,
-                           descs, b->d_items, lo, hi, b->d_bd);
+        if (b->split)
+            hipLaunchKernelGGL((k_march_batch<LIM, HLL, S>), dim3(G), dim3(tile_threads<float, S>()), 0, b->stream,
+                               descs, b->d_items, lo, hi, b->d_bd);
+        else
+            hipLaunchKernelGGL((k_fused_batch<LIM, HLL, S, ALT>), dim3(G), dim3(tile_threads<float, S>()), 0, b->stream,
+                               descs, b->d_items, lo, hi, b->d_bd);
     }
 }
 
@@ -450,6 +506,8 @@ void launch_stage(cylmhd_batch *b, int s)
     launch_shape<LIM, HLL, 1, ALT>(b, descs);
     launch_shape<LIM, HLL, 2, ALT>(b, descs);
     launch_shape<LIM, HLL, 3, ALT>(b, descs);
+    if (b->split)
+        hipLaunchKernelGGL((k_update_batch<ALT>), dim3(b->ublocks), dim3(256), 0, b->stream, descs, b->d_uoff, b->n, b->d_bd);
     hipLaunchKernelGGL(k_halo_batch, dim3(b->hblocks), dim3(256), 0, b->stream, descs, b->d_hoff, b->n, b->d_bd);
 }
 
@@ -487,8 +545,8 @@ int prime(cylmhd_batch *b)
         BCHK(hipStreamSynchronize(b->view[d].stream));
         BCHK(hipMemsetAsync(b->view[d].ds.counts, 0, 9 * sizeof(int), b->stream));
     }
-    hipLaunchKernelGGL(k_setdt_batch, dim3(b->nitems), dim3(256), 0, b->stream, b->d_descs, b->d_bd, b->d_ioff, b->n,
-                       b->d_dtpart);
+    hipLaunchKernelGGL(k_setdt_batch, dim3(b->split ? b->ublocks : b->nitems), dim3(256), 0, b->stream, b->d_descs, b->d_bd,
+                       b->split ? b->d_uoff : b->d_ioff, b->n, b->d_dtpart);
     BCHK(hipGetLastError());
     b->primed = true;
     return CYLMHD_OK;
@@ -536,6 +594,12 @@ int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out)
     }
     cylmhd_batch *b = new cylmhd_batch();
     b->n = n;
+    {
+        /* the stage form until the tuner picks one: two kernels, or one
+         * with CYLMHD_STAGE_KERNELS=1 in the environment */
+        const char *e = getenv("CYLMHD_STAGE_KERNELS");
+        b->split = !(e && e[0] == '1');
+    }
     b->dshape.assign(n, 0);
     b->lim = p[0].limiter;
     for (int d = 0; d < n; d++) b->alt = b->alt || p[d].source != CYLMHD_SOURCE_SPIRAL_COOLING;
@@ -656,19 +720,23 @@ int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
     std::vector<int> avail;
     for (int sh = 0; sh < kShapes; sh++)
         if (sh < 2 || ((b->lim == 0 || b->lim == 1) && !b->hll)) avail.push_back(sh);
-    /* 1: one shape for every disc x segment length x grid */
+    /* 1: the stage form x one shape for every disc x segment length x grid */
     double best = 1e300;
     int bs = 0, bl = 0, bg = 1024;
-    for (int sh : avail)
-        for (int li = 0; li < 4; li++)
-            for (int gi = 0; gi < 2; gi++) {
-                b->dshape.assign(b->n, sh);
-                b->seglen = seglens[li];
-                b->grid = grids[gi];
-                float f = 0.0f;
-                if ((rc = time_it(&f))) return rc;
-                if (f < best) { best = f; bs = sh; bl = seglens[li]; bg = grids[gi]; }
-            }
+    bool bf = true;
+    for (int form = 1; form >= 0; form--)
+        for (int sh : avail)
+            for (int li = 0; li < 4; li++)
+                for (int gi = 0; gi < 2; gi++) {
+                    b->split = form == 1;
+                    b->dshape.assign(b->n, sh);
+                    b->seglen = seglens[li];
+                    b->grid = grids[gi];
+                    float f = 0.0f;
+                    if ((rc = time_it(&f))) return rc;
+                    if (f < best) { best = f; bs = sh; bl = seglens[li]; bg = grids[gi]; bf = form == 1; }
+                }
+    b->split = bf;
     b->dshape.assign(b->n, bs);
     b->seglen = bl;
     b->grid = bg;
@@ -705,7 +773,7 @@ int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
     /* the floor counters of the timed candidates are not the discs' */
     for (int d = 0; d < b->n; d++) BCHK(hipMemsetAsync(b->view[d].ds.counts, 0, 9 * sizeof(int), b->stream));
     BCHK(hipStreamSynchronize(b->stream));
-    if (choice) { choice[0] = bs; choice[1] = bl; choice[2] = bg; }
+    if (choice) { choice[0] = bs; choice[1] = bl; choice[2] = bg; choice[3] = bf ? 2 : 1; }
     return CYLMHD_OK;
 }
 

@@ -58,7 +58,7 @@ __global__ __launch_bounds__((tile_threads<T, 0>()), (march_waves<T>())) void k_
 template <typename T, bool ALT>
 __global__ __launch_bounds__(256) void k_update(FArgs<T> a, int nch)
 {
-    update_cell<T, ALT>(a, nch);
+    update_cell<T, ALT>(a, nch, blockIdx.x, blockIdx.x);
 }
 
 /* blocks of the split stage's update: four wavefronts of 64 k each */
@@ -326,7 +326,8 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
         const char *e = getenv("CYLMHD_STAGE_KERNELS");
         fs.split = !(e && e[0] == '1');
     }
-    if (fs.split && (rc = alloc(9 * NS * fs.esize, &fs.div))) return rc;
+    /* the divergence of the two-kernel stage (a batch's discs always run it) */
+    if ((fs.split || fs.zg) && (rc = alloc(9 * NS * fs.esize, &fs.div))) return rc;
     const int TJ = Tile<T, 0>::TJ;
     const int TK = Tile<T, 0>::TK;
     fs.tiles_j = (g.NY - 2 + TJ - 1) / TJ;

@@ -905,8 +905,9 @@ __device__ __forceinline__ void fused_tile(const FArgs<T> &a, int tile, int seg,
  * faces load their two states into registers, a barrier, then the phi
  * faces write their fluxes and the z states are written over the phi
  * states (one barrier more per plane).
- * Results are those of fused_tile bit for bit. */
-template <typename T, int LIM, bool HLL, int SHAPE>
+ * Results are those of fused_tile bit for bit.  ZGL: the batch layout,
+ * whose z-side ghost cells sit in packed planes after the state body. */
+template <typename T, int LIM, bool HLL, int SHAPE, bool ZGL = false>
 __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
 {
     constexpr int TJ = Tile<T, SHAPE>::TJ, TK = Tile<T, SHAPE>::TK;
@@ -979,7 +980,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
 #pragma unroll
         for (int m = 0; m < 9; m++) plane[off + m * PS] = v[m];
     };
-    auto col_V = [&](int p, T *v) { cell_V(a, SI(p, cjm, ckm), p, ckm, v); };
+    auto col_V = [&](int p, T *v) { cell_V_at<ZGL>(a, p, cjm, ckm, v); };
     int hj = 0, hk = 0;
     if (t < 2 * TK) { hj = -2 + t / TK; hk = t % TK; }
     else if (t < 4 * TK) { hj = TJ + (t - 2 * TK) / TK; hk = (t - 2 * TK) % TK; }
@@ -1021,14 +1022,19 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
         const bool pre = i + 2 <= g.NX - 1;
         T raw[9], rub[NOFF];
         if (pre && is_cell) {
-            const size_t cp = SI(i + 2, cjm, ckm);
+            /* ZGL: past the end of a partial tile the own column is the
+             * packed z-side ghost */
+            const bool zg = ZGL && ckm == g.NZ - 1;
+            const size_t cp = zg ? ZG(1, i + 2, cjm) : SI(i + 2, cjm, ckm);
+            const size_t st = zg ? (size_t)1 : NS;
             ub_load(a, i + 2, ckm, rub);
 #pragma unroll
-            for (int m = 0; m < 9; m++) raw[m] = a.Uin[m * NS + cp];
+            for (int m = 0; m < 9; m++) raw[m] = a.Uin[m * st + cp];
         }
         if (t < NHALO) {
             T v[9];
-            cell_V(a, (size_t)i * g.NY * a.NZP + hcol, i, kmap(g, k0 + hk), v);
+            if constexpr (ZGL) cell_V_at<true>(a, i, jmap(g, j0 + hj), kmap(g, k0 + hk), v);
+            else cell_V(a, (size_t)i * g.NY * a.NZP + hcol, i, kmap(g, k0 + hk), v);
             st9(hoff, v);
         }
         {
@@ -1117,7 +1123,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                     T vm[9], v0[9], vp[9];
                     const int jj = jmap(g, j0 + fzj), L = g.NZ - 2;
                     cell_V(a, SI(i, jj, L - 1), i, L - 1, vm); cell_V(a, SI(i, jj, L), i, L, v0);
-                    cell_V(a, SI(i, jj, L + 1), i, L + 1, vp);
+                    cell_V_at<ZGL>(a, i, jj, L + 1, vp);
                     cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), vl, (T *)nullptr);
                 } else {
                     T vm[9], v0[9], vp[9];
@@ -1137,7 +1143,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                 } else if (LIM == 3) {
                     T vm[9], v0[9], vp[9];
                     const int jj = jmap(g, j0 + fzj);
-                    cell_V(a, SI(i, jj, 0), i, 0, vm);
+                    cell_V_at<ZGL>(a, i, jj, 0, vm);
                     cell_V(a, SI(i, jj, 1), i, 1, v0); cell_V(a, SI(i, jj, 2), i, 2, vp);
                     cell_states<2, LIM>(c, vm, v0, vp, T(1.0), T(1.0), T(1.0), (T *)nullptr, vr);
                 } else {
@@ -1213,22 +1219,42 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
  * wavefront over 64 consecutive k of one (i, j) row (chunk `ch` of the
  * row's nch = ceil((NZ-2)/64)), four wavefronts to a block; the stage
  * input is read once (UtoV gives the primitives the sources use, as the
- * plane loads of the march do), the divergence from a.div.  Floor counters
- * and the Setdt partial (slot blockIdx.x) as in fused_tile. */
-template <typename T, bool ALT>
-__device__ __forceinline__ void update_cell(const FArgs<T> &a, int nch)
+ * plane loads of the march do), the divergence from a.div.  blk: the
+ * block's index among this disc's update blocks of the stage.  FLAT: the
+ * cells are dealt 256 to a block in (i, j, k) order instead (discs whose
+ * rows are short against a wavefront).  Floor counters and the Setdt
+ * partial (dtpart slot `slot`) as in fused_tile. */
+template <typename T, bool ALT, bool FLAT = false>
+__device__ __forceinline__ void update_cell(const FArgs<T> &a, int nch, int blk, int slot)
 {
     const DevGeom &g = a.g;
     const size_t NS = a.NS;
     const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const int nj = g.NY - 2, nrows = (g.NX - 2) * nj;
-    const int r = w / nch, k = 1 + (w - r * nch) * 64 + lane;
+    int i, j, k;
+    bool on;
+    if constexpr (FLAT) {
+        /* the interior cells in (i, j, k) order, k fastest, 64 to a wave
+         * whatever NZ (short rows fill whole waves; i may vary in a wave) */
+        const unsigned nk = (unsigned)(g.NZ - 2);
+        const unsigned q = (unsigned)blk * blockDim.x + threadIdx.x;
+        const unsigned r = q / nk;
+        on = r < (unsigned)nrows;
+        k = 1 + (int)(q - r * nk);
+        i = 1 + (int)(r / (unsigned)nj);
+        j = 1 + (int)(r % (unsigned)nj);
+    } else {
+        const int w = __builtin_amdgcn_readfirstlane((int)((blk * blockDim.x + threadIdx.x) >> 6));
+        const int r = w / nch;
+        k = 1 + (w - r * nch) * 64 + lane;
+        on = r < nrows && k <= g.NZ - 2;
+        i = 1 + r / nj;
+        j = 1 + r % nj;
+    }
     int f0 = 0, f1 = 0, f2 = 0;
     double best = 0.0;
     long long bix = LLONG_MAX;
-    if (r < nrows && k <= g.NZ - 2) {
-        const int i = 1 + r / nj, j = 1 + r % nj;
+    if (on) {
         const cdp it = (cdp)a.itab + (size_t)i * IT;
         const double sc[2] = {ro(g.sincos, (size_t)i * g.NY + j), ro(g.sincos, (size_t)g.NX * g.NY + (size_t)i * g.NY + j)};
         const size_t cc = SI(i, j, k), ccr = C3(i, j, k), ik = (size_t)i * g.NZ + k;
@@ -1270,7 +1296,7 @@ __device__ __forceinline__ void update_cell(const FArgs<T> &a, int nch)
         __shared__ double sv[4];
         __shared__ long long sx[4];
         block_dt_max<4>(best, bix, sv, sx);
-        if (threadIdx.x == 0) { a.dtpart[2 * blockIdx.x] = best; a.dtpart[2 * blockIdx.x + 1] = (double)bix; }
+        if (threadIdx.x == 0) { a.dtpart[2 * slot] = best; a.dtpart[2 * slot + 1] = (double)bix; }
     }
 }
 

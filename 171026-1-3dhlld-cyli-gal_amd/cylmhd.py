@@ -421,7 +421,7 @@ class Batch:
         return _DiscRef(self.params[d], h)
 
     def tune(self, steps=2):
-        ch = (C.c_int * 3)()
+        ch = (C.c_int * 4)()
         check(lib().cylmhd_batch_tune(self.h, steps, ch), "batch_tune")
         return tuple(ch)
 

@@ -224,15 +224,18 @@ int cylmhd_batch_destroy(cylmhd_batch *b);
  * dumps, slices and diagnostics work on it; cylmhd_step does not (the batch
  * keeps its own state layout): cylmhd_batch_run advances it. */
 int cylmhd_batch_disc(cylmhd_batch *b, int d, cylmhd_ctx **ctx);
-/* Autotuner: times `steps` steps for each tile shape x r-segment length x
- * launch grid with one tile for every disc, then, per class of disc size
- * (the discs of one phi x z size), the other tiles for that class; works on
- * the current states and each disc's time, steps and alive flag, puts them
- * back after every candidate and keeps the fastest.  Tiles (phi x z cells,
- * threads, i.e. waves per tile): 0: 14x16, 256; 1: 6x32, 256; 2: 6x16,
- * 128; 3: 10x16, 192 (2 and 3 for minmod and MC with HLLD only).  choice (optional, 3 ints) = the common
- * tile of the first pass, segment length (0 = whole r), grid. */
-int cylmhd_batch_tune(cylmhd_batch *b, int steps, int choice[3]);
+/* Autotuner: times `steps` steps for each stage form x tile shape x
+ * r-segment length x launch grid with one tile for every disc, then, per
+ * class of disc size (the discs of one phi x z size), the other tiles for
+ * that class; works on the current states and each disc's time, steps and
+ * alive flag, puts them back after every candidate and keeps the fastest.
+ * Stage forms: one kernel per stage (each work item does the whole stage)
+ * or two (the r-march per work item, then one per-cell update over every
+ * disc).  Tiles (phi x z cells, threads, i.e. waves per tile): 0: 14x16,
+ * 256; 1: 6x32, 256; 2: 6x16, 128; 3: 10x16, 192 (2 and 3 for minmod and
+ * MC with HLLD only).  choice (optional, 4 ints) = the common tile of the
+ * first pass, segment length (0 = whole r), grid, kernels per stage (1, 2). */
+int cylmhd_batch_tune(cylmhd_batch *b, int steps, int choice[4]);
 /* The tile disc d runs with: tile[0..3] = shape, phi cells, z cells, threads. */
 int cylmhd_batch_disc_tile(cylmhd_batch *b, int d, int tile[4]);
 /* Advance every live disc nsteps steps (graph replays); *ms (optional) =

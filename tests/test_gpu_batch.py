@@ -79,6 +79,7 @@ def test_batch_tuner_keeps_the_states(gpu):
         choice = b.tune(2)
         print("tuned", choice)
         assert choice[0] in (0, 1, 2, 3) and choice[1] in (0, 8, 16, 32) and choice[2] in (512, 1024)
+        assert choice[3] in (1, 2)
         assert all(b.disc_tile(d)[0] in (0, 1, 2, 3) for d in range(len(ps)))
         b.run(3)
         tuned = [b.disc(d).download()[0] for d in range(len(ps))]
@@ -200,3 +201,21 @@ def test_batch_disc_is_not_stepped_alone(gpu):
         assert e.value.code == cm.EINVAL
         U, V = g.download()                  # set-up, dumps and diagnostics still work
         assert np.isfinite(U).all() and g.diagnostics()["mass"] > 0
+
+
+def test_batch_stage_forms_agree(gpu, monkeypatch):
+    """The stage as one kernel per work item and as two (the r-march per
+    work item, then the per-cell update over every disc) run the same
+    arithmetic: states, times and floor counts equal bit for bit."""
+    ps = batch_params()
+    out = {}
+    for form in ("1", "2"):
+        monkeypatch.setenv("CYLMHD_STAGE_KERNELS", form)
+        with start_batch(ps) as b:
+            b.run(4)
+            out[form] = [(b.status(d), b.disc(d).download()[0]) for d in range(len(ps))]
+    for d in range(len(ps)):
+        (s1, U1), (s2, U2) = out["1"][d], out["2"][d]
+        print(d, s1, s2, int(np.count_nonzero(U1 != U2)))
+        assert s1 == s2
+        np.testing.assert_array_equal(U1, U2)
