@@ -133,29 +133,28 @@ __device__ __forceinline__ void cell_states(const Consts &c, const T *vm, const 
     }
     const T dt = T(c.dt);
     if (sizeof(T) == 4 && (LIM == 0 || LIM == 1)) {
-        /* FP32 state: the same slopes as three-input min/max, which gfx950
-         * issues as one v_min3/v_max3 each.  For a > 0 the limited slope is
-         * max(0, min(a, b[, c])) and for a <= 0 min(0, max(a, b[, c])): with
-         * a b <= 0 both give 0, otherwise the reference's min/max chain
-         * (the MC centred slope c has the sign of a and b when they agree).
-         * Min and max round nothing, so the result equals the reference
-         * form except where a*b underflows. */
+        /* FP32 state: the same slopes as medians, one v_med3 each.
+         * minmod(a, b) = med3(a, b, 0): with a b <= 0 the median is 0,
+         * otherwise the one of smaller magnitude; and minmod(a, b, c) =
+         * minmod(minmod(a, b), c) (the MC centred slope c has the sign of a
+         * and b when they agree).  Medians round nothing, so the result
+         * equals the reference form except where a*b underflows (and in
+         * the sign of a zero slope). */
         const T hu = T(0.5) * (T(1.0) - lambmax * dt * f1hi * f2);
         const T hl = T(0.5) * (T(1.0) + lambmin * dt * f1lo * f2);
 #pragma unroll
         for (int m = 0; m < 9; m++) {
-            const T a = v0[m] - vm[m];
-            const T b = vp[m] - v0[m];
-            T grad;
+            const float a = v0[m] - vm[m];
+            const float b = vp[m] - v0[m];
+            const float mm = __builtin_amdgcn_fmed3f(a, b, 0.0f);
             if (LIM == 1) {
-                const T cc = T(0.25) * (vp[m] - vm[m]);
-                grad = a > T(0.0) ? fmaxf(0.0f, fminf(fminf(a, b), cc)) : fminf(0.0f, fmaxf(fmaxf(a, b), cc));
+                const float cc = 0.25f * (vp[m] - vm[m]);
+                const float grad = __builtin_amdgcn_fmed3f(mm, cc, 0.0f);
                 if (upper) upper[m] = v0[m] + grad;
                 if (lower) lower[m] = v0[m] - grad;
             } else {
-                grad = a > T(0.0) ? fmaxf(0.0f, fminf(a, b)) : fminf(0.0f, fmaxf(a, b));
-                if (upper) upper[m] = v0[m] + hu * grad;
-                if (lower) lower[m] = v0[m] - hl * grad;
+                if (upper) upper[m] = v0[m] + hu * mm;
+                if (lower) lower[m] = v0[m] - hl * mm;
             }
         }
         return;

@@ -917,7 +917,12 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
     static_assert(NZT <= NTH && TJ * (TK + 1) <= NTH && NHALO <= NTH, "tile does not fit the block");
     static_assert(TK <= NTH - NCELL && TK + 2 <= NTH, "no spare threads for the table rows");
     constexpr bool MC = LIM == 1;
-    constexpr int PK = TK + 4, PS = (TJ + 4) * PK, SLOT = 9 * PS;
+    /* FP32: component strides of the plane and of the z states rounded up
+     * to 64 words, so that one ds_read2st64 reads two components of a cell
+     * (FP64 keeps them tight: its LDS is at the four-blocks-per-CU limit) */
+    constexpr int PK = TK + 4, PSR = (TJ + 4) * PK;
+    constexpr int PS = sizeof(T) == 4 ? (PSR + 63) / 64 * 64 : PSR, SLOT = 9 * PS;
+    constexpr int NZS = sizeof(T) == 4 ? (NZT + 63) / 64 * 64 : NZT;
     const DevGeom &g = a.g;
     const Consts &c = a.c;
     /* plane i, padded as a ring slot of fused_tile */
@@ -927,11 +932,11 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
      * over the lower phi states) and the z states [2][9][NZT] */
     constexpr bool SHARE = sizeof(T) == 8;
     constexpr int FYN = (TJ + 1) * TK;
-    constexpr int SBUF = SHARE ? (2 * 9 * NTH > 9 * FYN + 2 * 9 * NZT ? 2 * 9 * NTH : 9 * FYN + 2 * 9 * NZT)
-                               : 2 * 9 * NTH + 2 * 9 * NZT;
+    constexpr int SBUF = SHARE ? (2 * 9 * NTH > 9 * FYN + 2 * 9 * NZS ? 2 * 9 * NTH : 9 * FYN + 2 * 9 * NZS)
+                               : 2 * 9 * NTH + 2 * 9 * NZS;
     __shared__ __align__(16) T sbuf[SBUF];
     T *const sy0 = sbuf, *const sy1 = sbuf + 9 * NTH;
-    T *const sz0 = SHARE ? sbuf + 9 * FYN : sbuf + 2 * 9 * NTH, *const sz1 = sz0 + 9 * NZT;
+    T *const sz0 = SHARE ? sbuf + 9 * FYN : sbuf + 2 * 9 * NTH, *const sz1 = sz0 + 9 * NZS;
     /* phi flux of face fyj (between rows fyj - 1 and fyj) at column fyk:
      * pf[m * PFS + (fyj + FOFF) * TK + fyk] */
     T *const pf = SHARE ? sbuf : sy1;
@@ -1069,7 +1074,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                 ld9(zbase, vm); ld9(zbase + 1, v0); ld9(zbase + 2, vp);
                 cell_states<2, LIM>(c, vm, v0, vp, s_odz[zc + 1][0], s_odz[zc + 1][1], T(1.0), up, lo);
 #pragma unroll
-                for (int m = 0; m < 9; m++) { sz0[m * NZT + t] = up[m]; sz1[m * NZT + t] = lo[m]; }
+                for (int m = 0; m < 9; m++) { sz0[m * NZS + t] = up[m]; sz1[m * NZS + t] = lo[m]; }
             }
         };
         if (!SHARE) z_states();
@@ -1118,7 +1123,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                     for (int m = 0; m < 9; m++) vl[m] = s_vz[0][fzj][m];
                 } else if (LIM == 2) {
 #pragma unroll
-                    for (int m = 0; m < 9; m++) vl[m] = sz1[m * NZT + ql];
+                    for (int m = 0; m < 9; m++) vl[m] = sz1[m * NZS + ql];
                 } else if (LIM == 3) {
                     T vm[9], v0[9], vp[9];
                     const int jj = jmap(g, j0 + fzj), L = g.NZ - 2;
@@ -1134,7 +1139,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                 }
             } else {
 #pragma unroll
-                for (int m = 0; m < 9; m++) vl[m] = sz0[m * NZT + qu];
+                for (int m = 0; m < 9; m++) vl[m] = sz0[m * NZS + qu];
             }
             if (kf == g.NZ - 2) {
                 if (MC) {
@@ -1152,11 +1157,11 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                 }
             } else {
 #pragma unroll
-                for (int m = 0; m < 9; m++) vr[m] = sz1[m * NZT + ql];
+                for (int m = 0; m < 9; m++) vr[m] = sz1[m * NZS + ql];
             }
             riemann<2, HLL>(c, vl, vr, T(it[IT_X]), T(it[IT_OX]), fz);
 #pragma unroll
-            for (int m = 0; m < 9; m++) sz1[m * NZT + ql] = fz[m];
+            for (int m = 0; m < 9; m++) sz1[m * NZS + ql] = fz[m];
         }
         /* ---- x: r-states of cell i+1 (planes i, i+1, i+2 of the own
          * column) and the flux at i+1/2 ----------------------------------- */
@@ -1197,7 +1202,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
                 const T g5 = s_g[sl][ck][3], g6 = g5;
                 const int qm = cj * (TK + 2) + ck + 1, qp = qm + 1;
 #pragma unroll
-                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz1[m * NZT + qp] - g5 * sz1[m * NZT + qm]);
+                for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz1[m * NZS + qp] - g5 * sz1[m * NZS + qm]);
             }
             const size_t cc = SI(i, gj, gk);
 #pragma unroll
