@@ -38,7 +38,7 @@ struct FusedState {
     int ncand = 0, cand[8] = {0};
     bool tuned = false;
     /* padded layout of the state arrays: component stride NS, k rows of
-     * NZP cells with k = 1 at offset 16 (64-byte aligned FP32 tile rows) */
+     * NZP cells with k = 1 at offset 32 (128-byte aligned FP32 rows) */
     size_t NS = 0;
     int NZP = 0, NZ = 0;
     /* batch layout (discs of a batch only): the z-side ghost cells (k = 0,

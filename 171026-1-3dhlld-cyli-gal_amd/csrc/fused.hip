@@ -308,9 +308,9 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     fs.esize = sizeof(T);
     const size_t N = d.N;
     /* padded state layout: rows of NZP >= NZ + KOFF cells, NZP a multiple
-     * of 16, so that k = 1 of every row starts a 64-byte line */
+     * of 32, so that k = 1 of every row starts a 128-byte line */
     fs.NZ = g.NZ;
-    fs.NZP = (g.NZ + KOFF + 15) / 16 * 16;
+    fs.NZP = (g.NZ + KOFF + 31) / 32 * 32;
     fs.NS = (size_t)g.NX * g.NY * fs.NZP;
     const size_t NS = fs.NS;
     int rc;

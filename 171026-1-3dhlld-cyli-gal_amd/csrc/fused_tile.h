@@ -55,12 +55,15 @@ namespace FUSED_NS {
 
 using FT = FUSED_FT;
 
-constexpr int KOFF = 15;     /* k = 1 at offset 16 of a padded row */
+constexpr int KOFF = 31;     /* k = 1 at offset 32 of a padded row */
 
 /* C3: cell index in the reference layout (Uinitial, the dt argmax);
  * SI: cell index in the state arrays of this path, whose k rows are padded
- * so that k = 1 (the first interior cell) starts a 64-byte line: a tile row
- * of 16 FP32 cells is then one aligned 64-byte access. */
+ * so that k = 1 (the first interior cell) starts a 128-byte line: a tile row
+ * of 16 FP32 cells is then one aligned 64-byte access, and a wavefront's 64
+ * consecutive interior k are two whole 128-byte lines (with k = 1 at offset
+ * 16, the interior of a 256-cell row spanned nine lines, two of them half
+ * padding). */
 #define C3(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * g.NZ + (size_t)(k))
 #define SI(i, j, k) ((((size_t)(i)) * g.NY + (size_t)(j)) * (size_t)a.NZP + (size_t)(k) + KOFF)
 /* ZG: with the batch layout the z-side ghost cells (k = 0: side 0, k = NZ-1:
