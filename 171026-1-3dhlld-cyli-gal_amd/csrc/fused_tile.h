@@ -1232,7 +1232,7 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
  * cells are dealt 256 to a block in (i, j, k) order instead (discs whose
  * rows are short against a wavefront).  Floor counters and the Setdt
  * partial (dtpart slot `slot`) as in fused_tile. */
-template <typename T, bool ALT, bool FLAT = false>
+template <typename T, bool ALT, bool FLAT = false, int RJ = 1>
 __device__ __forceinline__ void update_cell(const FArgs<T> &a, int nch, int blk, int slot)
 {
     const DevGeom &g = a.g;
@@ -1252,30 +1252,40 @@ __device__ __forceinline__ void update_cell(const FArgs<T> &a, int nch, int blk,
         i = 1 + (int)(r / (unsigned)nj);
         j = 1 + (int)(r % (unsigned)nj);
     } else {
+        /* RJ consecutive rows j of one plane per wavefront (nphi is even:
+         * the axis map j <-> j + nphi/2 needs it) */
+        const int njb = (nj + RJ - 1) / RJ;
         const int w = __builtin_amdgcn_readfirstlane((int)((blk * blockDim.x + threadIdx.x) >> 6));
         const int r = w / nch;
         k = 1 + (w - r * nch) * 64 + lane;
-        on = r < nrows && k <= g.NZ - 2;
-        i = 1 + r / nj;
-        j = 1 + r % nj;
+        on = r < (g.NX - 2) * njb && k <= g.NZ - 2;
+        i = 1 + r / njb;
+        j = 1 + (r % njb) * RJ;
     }
     int f0 = 0, f1 = 0, f2 = 0;
     double best = 0.0;
     long long bix = LLONG_MAX;
     if (on) {
         const cdp it = (cdp)a.itab + (size_t)i * IT;
-        const double sc[2] = {ro(g.sincos, (size_t)i * g.NY + j), ro(g.sincos, (size_t)g.NX * g.NY + (size_t)i * g.NY + j)};
-        const size_t cc = SI(i, j, k), ccr = C3(i, j, k), ik = (size_t)i * g.NZ + k;
-        T uin[9], unn[9], acc[9], v[9], ub[NOFF];
+        const size_t ik = (size_t)i * g.NZ + k;
+        /* every load of the RJ cells issued before the first is used; the
+         * (i, k) and k table rows serve all of them */
+        T uin[RJ][9], unn[RJ][9], acc[RJ][9], rfl[RJ], ub[NOFF];
+        bool row[RJ];
 #pragma unroll
-        for (int m = 0; m < 9; m++) { uin[m] = a.Uin[m * NS + cc]; unn[m] = T(0); }
-        if (a.kind == 2 || a.kind == 3 || a.kind == 5) {
+        for (int q = 0; q < RJ; q++) {
+            row[q] = j + q <= g.NY - 2;
+            const size_t cc = SI(i, row[q] ? j + q : j, k);
 #pragma unroll
-            for (int m = 0; m < 9; m++) unn[m] = a.Un[m * NS + cc];
+            for (int m = 0; m < 9; m++) { uin[q][m] = a.Uin[m * NS + cc]; unn[q][m] = T(0); }
+            if (a.kind == 2 || a.kind == 3 || a.kind == 5) {
+#pragma unroll
+                for (int m = 0; m < 9; m++) unn[q][m] = a.Un[m * NS + cc];
+            }
+#pragma unroll
+            for (int m = 0; m < 9; m++) acc[q][m] = a.div[m * NS + cc];
+            rfl[q] = a.rhofloor[cc];
         }
-#pragma unroll
-        for (int m = 0; m < 9; m++) acc[m] = a.div[m * NS + cc];
-        const T rfl = a.rhofloor[cc];
         ub_load(a, i, k, ub);
         double ikd[4], kt[5], cabs = 0.0;
 #pragma unroll
@@ -1283,14 +1293,23 @@ __device__ __forceinline__ void update_cell(const FArgs<T> &a, int nch, int blk,
         const double *kr = a.ktab + (size_t)k * KT;
         kt[0] = kr[KT_SZS]; kt[1] = kr[KT_Z]; kt[2] = kr[KT_ONEPA]; kt[3] = kr[KT_OZ2S2]; kt[4] = kr[KT_ZABS];
         if (i <= g.irin) cabs = g.cabs[ik];
-        {
-            T u[9];
 #pragma unroll
-            for (int m = 0; m < 9; m++) u[m] = uin[m];
-            u[8] = T(a.c.psicoef) * u[8];
-            Prec<T>::utov(a.c, u, ub, it[IT_OX], v);
+        for (int q = 0; q < RJ; q++) {
+            if (!row[q]) continue;
+            const int jq = j + q;
+            const double sc[2] = {ro(g.sincos, (size_t)i * g.NY + jq), ro(g.sincos, (size_t)g.NX * g.NY + (size_t)i * g.NY + jq)};
+            const size_t cc = SI(i, jq, k), ccr = C3(i, jq, k);
+            T v[9];
+            {
+                T u[9];
+#pragma unroll
+                for (int m = 0; m < 9; m++) u[m] = uin[q][m];
+                u[8] = T(a.c.psicoef) * u[8];
+                Prec<T>::utov(a.c, u, ub, it[IT_OX], v);
+            }
+            cell_update<T, ALT>(a, i, jq, k, cc, ccr, it, acc[q], v, uin[q], unn[q], rfl[q], ikd, kt, sc, &cabs, ub, f0, f1,
+                                f2, best, bix);
         }
-        cell_update<T, ALT>(a, i, j, k, cc, ccr, it, acc, v, uin, unn, rfl, ikd, kt, sc, &cabs, ub, f0, f1, f2, best, bix);
     }
     for (int off = 32; off > 0; off >>= 1) {
         f0 += __shfl_xor(f0, off); f1 += __shfl_xor(f1, off); f2 += __shfl_xor(f2, off);
