@@ -228,7 +228,7 @@ int alloc_state(cylmhd_ctx *c)
         /* fused paths keep no V and no face arrays; V is only materialised
          * on demand (Setdt, download) into a scratch array */
         if ((rc = dalloc(c, &d.V, 9 * d.N))) return rc;
-        c->fs.zg = c->batch_layout;
+        c->fs.batch = c->batch_layout;
         if ((rc = fused_alloc(c->fs, c->dg, d, c->p.path == CYLMHD_PATH_FUSED32,
                               [c](size_t bytes, void **ptr) {
                                   char *q;

@@ -41,10 +41,13 @@ struct FusedState {
      * NZP cells with k = 1 at offset 32 (128-byte aligned FP32 rows) */
     size_t NS = 0;
     int NZP = 0, NZ = 0;
-    /* batch layout (discs of a batch only): the z-side ghost cells (k = 0,
-     * NZ-1) of each state buffer are not in its [9][NS] body but packed
-     * after it, [2][NX][NY][9]; nstate = elements per state buffer */
-    bool zg = false;
+    /* packed z-side ghosts (the discs of a batch, and a single disc on the
+     * two-kernel stage): the z-side ghost cells (k = 0, NZ-1) of each state
+     * buffer are not in its [9][NS] body but packed after it,
+     * [2][NX][NY][9], so the ghost pass writes 36 contiguous bytes per
+     * cell; nstate = elements per state buffer.  batch: a disc of a batch
+     * (advanced only by the batch) */
+    bool zg = false, batch = false;
     size_t nstate = 0;
     /* geometry packed for the stage kernel (few base pointers):
      *   itab [NX][IT] FP64  x, xm, oneox, oneodx, xabs, spiral r factors

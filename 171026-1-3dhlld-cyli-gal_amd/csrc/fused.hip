@@ -47,12 +47,13 @@ __global__ __launch_bounds__((tile_threads<T, 0>()), (tile_waves<T, 0>())) void 
  * per CU (51 KB of LDS each, <= 168 VGPRs), FP64 two waves (256 VGPRs) */
 template <typename T> constexpr int march_waves() { return sizeof(T) == 4 ? 3 : 2; }
 
+/* FP32: the z-side ghosts in packed planes (FusedState::zg) */
 template <typename T, int LIM, bool HLL>
 __global__ __launch_bounds__((tile_threads<T, 0>()), (march_waves<T>())) void k_march(FArgs<T> a)
 {
     const int b = remap_block(blockIdx.x, gridDim.x);
     if (b >= a.nblocks) return;
-    march_tile<T, LIM, HLL, 0>(a, b % a.ntiles, b / a.ntiles);
+    march_tile<T, LIM, HLL, 0, sizeof(T) == 4>(a, b % a.ntiles, b / a.ntiles);
 }
 
 /* rows j a wavefront of the update advances, their loads issued together
@@ -85,10 +86,10 @@ __global__ __launch_bounds__(256) void k_edge(FArgs<T> a, T *edge)
     edge_cell<T, LIM, HLL>(a, edge, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-template <typename T>
+template <typename T, bool ZGL>
 __global__ __launch_bounds__(256) void k_halo(FArgs<T> a)
 {
-    halo_range<T>(a, blockIdx.x * (size_t)blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
+    halo_range<T, ZGL>(a, blockIdx.x * (size_t)blockDim.x + threadIdx.x, (size_t)gridDim.x * blockDim.x);
 }
 
 /* ---- RK2 only: interior plane k = 1 gets U3 = 0 after the update (the
@@ -324,6 +325,17 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     fs.NS = (size_t)g.NX * g.NY * fs.NZP;
     const size_t NS = fs.NS;
     int rc;
+    if (fs.batch) {
+        fs.zg = true;                        /* a batch's discs are advanced by the batch */
+    } else {
+        const char *e = getenv("CYLMHD_STAGE_KERNELS");
+        fs.split = !(e && e[0] == '1');
+        /* the FP32 two-kernel stage reads and writes packed z-side ghosts
+         * (its ghost pass then writes 36 contiguous bytes per cell, not
+         * nine 64-byte lines); in the FP64 r-march the extra address
+         * selection spills registers */
+        fs.zg = fs.split && fs.fp32;
+    }
     fs.nstate = 9 * NS + (fs.zg ? 18 * (size_t)g.NX * g.NY : 0);
     if ((rc = alloc(fs.nstate * fs.esize, &fs.Un)) || (rc = alloc(fs.nstate * fs.esize, &fs.A)) ||
         (rc = alloc(fs.nstate * fs.esize, &fs.B)) || (rc = alloc((9 * NS + 27 * (size_t)g.NX * g.NY) * fs.esize, &fs.shadow)) ||
@@ -332,12 +344,8 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     void *p;
     if ((rc = alloc(9 * N * sizeof(double), &p))) return rc;
     fs.u64_scratch = (double *)p;
-    if (!fs.zg) {                             /* a batch's discs are advanced by the batch */
-        const char *e = getenv("CYLMHD_STAGE_KERNELS");
-        fs.split = !(e && e[0] == '1');
-    }
-    /* the divergence of the two-kernel stage (a batch's discs always run it) */
-    if ((fs.split || fs.zg) && (rc = alloc(9 * NS * fs.esize, &fs.div))) return rc;
+    /* the divergence of the two-kernel stage (a batch's discs may run it) */
+    if ((fs.split || fs.batch) && (rc = alloc(9 * NS * fs.esize, &fs.div))) return rc;
     const int TJ = Tile<T, 0>::TJ;
     const int TK = Tile<T, 0>::TK;
     fs.tiles_j = (g.NY - 2 + TJ - 1) / TJ;
@@ -553,14 +561,15 @@ static void stage_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
     }
     if (ev1) hipEventRecord(ev1, s);
     const size_t nghost = 2 * (size_t)g.NY * g.NZ + (size_t)(g.NX - 2) * 2 * g.NZ + (size_t)(g.NX - 2) * (g.NY - 2) * 2;
-    hipLaunchKernelGGL(k_halo<T>, dim3(nblocks(nghost, 256, 4096)), dim3(256), 0, s, a);
+    if (fs.zg) hipLaunchKernelGGL((k_halo<T, true>), dim3(nblocks(nghost, 256, 4096)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_halo<T, false>), dim3(nblocks(nghost, 256, 4096)), dim3(256), 0, s, a);
     if (last) fs.dt_valid = true;
 }
 
 int stage(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c,
           const StageArgs &a, hipEvent_t ev0, hipEvent_t evm, hipEvent_t ev1)
 {
-    if (fs.zg) return CYLMHD_EINVAL;   /* a batch's disc: the batch advances it */
+    if (fs.batch) return CYLMHD_EINVAL;   /* a batch's disc: the batch advances it */
     /* one kernel per limiter x solver; the default source (SpiralGravityCooling)
      * runs a kernel compiled for it alone */
 #define CYL_STAGE(L, H, A) stage_t<FT, L, H, A>(s, fs, g, d, c, a, ev0, evm, ev1)
