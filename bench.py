@@ -170,6 +170,12 @@ def run_batch(cm, steps, n=48, seed=20171026):
                       "segment": choice[1] or "whole r", "grid": choice[2], "kernels_per_stage": choice[3],
                       "tile_by_class_nphi_x_nz": tiles, "tune_s": tune_s})
     log(f"[bench] batch: {res['value']:.4g} cell-updates/s over {steps} steps ({ms:.1f} ms), tuned {res['tuned']}")
+    # the batch's scheme (minmod + HLLD, the batch discs' parameters) on one
+    # 512x512x256 disc, the single-disc rate the batch is compared with
+    import numpy as np
+    one = run_path(cm, np, cm.PATH_FUSED32, (512, 512, 256), 6, 1, config=4)
+    res["single_disc_same_scheme"] = dict(grid=[512, 512, 256], value=one["value"], ms_per_step=one["ms_per_step"])
+    res["over_single_disc"] = res["value"] / one["value"]
     return res
 
 
