@@ -61,7 +61,7 @@ extern "C" {
 #define CYLMHD_GRAV_POINTMASS 1   /* -GM r_vec/|r|^3 (Keplerian discs)       */
 /* device path                                                             */
 #define CYLMHD_PATH_PLAIN64   0   /* pass-by-pass FP64, reference data flow  */
-#define CYLMHD_PATH_FUSED64   1   /* one fused kernel per RK stage, FP64     */
+#define CYLMHD_PATH_FUSED64   1   /* fused RK stage (r-march + update), FP64 */
 #define CYLMHD_PATH_FUSED32   2   /* fused, FP32 state + FP64 recovery/sums  */
 
 /* ---- run-time parameter set: input.h:45-102 + constant.h:62-89 --------- */
