@@ -95,3 +95,33 @@ def test_split_segments(gpu, path):
     # several segmentations; every choice must give the same states
     p, Ui, Vi, Uc, Vc = disc_case(3, 200, 30, 34)
     assert_same(p, Ui, Vi, Uc, Vc, 2, path)
+
+
+@pytest.mark.parametrize("path", [cm.PATH_FUSED32, cm.PATH_FUSED64, cm.PATH_PLAIN64], ids=["fused32", "fused64", "plain64"])
+def test_kernel_timing_split(gpu, path):
+    """cylmhd_kernel_timing: which = 1 / 2 time the r-march and the update of
+    the two-kernel stage apart (their sum is the whole stage, which = 0);
+    the plain path has no such split (no launches reported)."""
+    p, Ui, Vi, Uc, Vc = disc_case(2, 64, 32, 32)
+    q = p.copy()
+    q.path = path
+    with cm.Context(q) as g:
+        g.set_initial(Ui, Vi)
+        g.kernel_timing(0, reset=True)
+        t = 0.0
+        for _ in range(3):
+            dt, ch, _ = g.setdt()
+            g.gravity(t)
+            g.step(dt, ch)
+            t += dt
+        m_ms, m_n = g.kernel_timing(1)
+        u_ms, u_n = g.kernel_timing(2)
+        s_ms, s_n = g.kernel_timing(0, reset=True)
+        print(path, s_ms, s_n, m_ms, m_n, u_ms, u_n)
+        assert s_n == 9 and s_ms > 0
+        if path == cm.PATH_PLAIN64:
+            assert m_n == 0 and u_n == 0
+        else:
+            assert m_n == 9 and u_n == 9 and m_ms > 0 and u_ms > 0
+            assert abs((m_ms + u_ms) - s_ms) <= 0.05 * s_ms + 0.01
+        assert g.kernel_timing(0)[1] == 0          # reset
