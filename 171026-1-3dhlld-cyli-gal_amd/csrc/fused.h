@@ -24,6 +24,7 @@ struct FusedState {
     void *edge = nullptr;      /* [2][9][NY][NZ] x-boundary face fluxes */
     double *u64_scratch = nullptr;   /* [9][N] FP64 export buffer */
     double *dtpart = nullptr;  /* Setdt partials of the last stage-3 launch */
+    double *dtpart2 = nullptr; /* [kDtSlices][2]: the partials reduced per slice */
     int dt_blocks = 0;
     int dt_nb = 0;             /* partials the last stage wrote */
     /* the stage as two kernels, the r-march (writing each cell's

@@ -127,22 +127,30 @@ __global__ __launch_bounds__(256) void k_setdt_state(FArgs<T> a)
 }
 
 /* the Setdt partials (up to one per block of the last stage's update
- * kernel: tens of thousands on the split stage) -> dt and its cell; four
- * partials per thread and round, loaded before any is compared, so that
- * the loads are in flight together */
-__global__ __launch_bounds__(1024) void k_dt_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+ * kernel: a hundred thousand on the split stage) -> dt and its cell, in
+ * two launches where there are many: block b reduces the partials of
+ * slice b (of `per` each) into partial b of out (same encoding: value,
+ * index, an index >= 9e18 for none), then one block reduces those into
+ * (dt-rate, cell, or -1 for none).  Four partials per thread and round,
+ * loaded before any is compared, so that the loads are in flight
+ * together. */
+constexpr int kDtSlices = 256;
+
+__global__ __launch_bounds__(1024) void k_dt_final(const double *__restrict__ part, int nb, int per, bool last,
+                                                   double *__restrict__ out)
 {
     __shared__ double sv[16];
     __shared__ long long sx[16];
     const int t = threadIdx.x;
+    const int lo = blockIdx.x * per, hi = min(nb, lo + per);
     double best = 0.0;
     long long bix = LLONG_MAX;
-    for (int b0 = t; b0 < nb; b0 += 4 * 1024) {
+    for (int b0 = lo + t; b0 < hi; b0 += 4 * 1024) {
         double2 q[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
             const int b = b0 + u * 1024;
-            q[u] = b < nb ? ((const double2 *)part)[b] : make_double2(0.0, 9.3e18);
+            q[u] = b < hi ? ((const double2 *)part)[b] : make_double2(0.0, 9.3e18);
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -152,7 +160,10 @@ __global__ __launch_bounds__(1024) void k_dt_final(const double *__restrict__ pa
         }
     }
     block_dt_max<16>(best, bix, sv, sx);
-    if (t == 0) { out[0] = best; out[1] = bix == LLONG_MAX ? -1.0 : (double)bix; }
+    if (t == 0) {
+        out[2 * blockIdx.x] = best;
+        out[2 * blockIdx.x + 1] = last ? (bix == LLONG_MAX ? -1.0 : (double)bix) : (double)bix;
+    }
 }
 
 /* ---- conversions and diagnostics ---------------------------------------- */
@@ -400,6 +411,8 @@ int alloc(FusedState &fs, const DevGeom &g, const DevState &d, AllocFn alloc)
     fs.dt_nb = (cols * fs.nseg + 7) / 8 * 8;
     if ((rc = alloc(2 * (size_t)fs.dt_blocks * sizeof(double), &p))) return rc;
     fs.dtpart = (double *)p;
+    if ((rc = alloc(2 * (size_t)kDtSlices * sizeof(double), &p))) return rc;
+    fs.dtpart2 = (double *)p;
     const size_t nik = (size_t)g.NX * g.NZ;
     if ((rc = alloc((size_t)g.NX * IT * sizeof(double), &p))) return rc;
     fs.itab = (double *)p;
@@ -472,7 +485,14 @@ static void setdt_t(hipStream_t s, FusedState &fs, const DevGeom &g, const DevSt
     } else {
         nb = fs.dt_nb;
     }
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1024), 0, s, fs.dtpart, nb, d.red);
+    if (nb > 4 * 1024) {
+        const int per = (nb + kDtSlices - 1) / kDtSlices;
+        const int ns = (nb + per - 1) / per;
+        hipLaunchKernelGGL(k_dt_final, dim3(ns), dim3(1024), 0, s, fs.dtpart, nb, per, false, fs.dtpart2);
+        hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1024), 0, s, fs.dtpart2, ns, ns, true, d.red);
+    } else {
+        hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(1024), 0, s, fs.dtpart, nb, nb, true, d.red);
+    }
 }
 
 int setdt(hipStream_t s, FusedState &fs, const DevGeom &g, const DevState &d, const Consts &c)
