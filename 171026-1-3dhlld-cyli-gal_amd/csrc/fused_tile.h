@@ -217,22 +217,43 @@ __device__ __forceinline__ void ub_load(const FArgs<T> &a, int i, int k, T *ub)
     }
 }
 
+/* Element idx of component m of a state-layout array (idx < NS): the
+ * component base arr + m NS is uniform and idx a 32-bit byte offset (a
+ * component stays below 4 GiB), so the access takes the scalar-base form
+ * with one 32-bit offset for all nine components instead of a 64-bit
+ * address per component and lane. */
+template <typename X>
+__device__ __forceinline__ X *comp_at(X *arr, size_t mNS, unsigned byte_off)
+{
+    return (X *)((char *)(arr + mNS) + byte_off);
+}
+
 /* stage input of one cell (i, k) as primitives: psi damped
  * (GLMSourceSolver) then UtoV */
 template <typename T>
-__device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, int k, T *v, size_t stride)
+__device__ __forceinline__ void cell_V_u(const FArgs<T> &a, T *u, int i, int k, T *v)
 {
-    T u[9], ub[NOFF];
-#pragma unroll
-    for (int m = 0; m < 9; m++) u[m] = a.Uin[m * stride + cc];
+    T ub[NOFF];
     ub_load(a, i, k, ub);
     u[8] = T(a.c.psicoef) * u[8];
     Prec<T>::utov(a.c, u, ub, ro(a.itab, (size_t)i * IT + IT_OX), v);
 }
 template <typename T>
+__device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, int k, T *v, size_t stride)
+{
+    T u[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) u[m] = a.Uin[m * stride + cc];
+    cell_V_u(a, u, i, k, v);
+}
+template <typename T>
 __device__ __forceinline__ void cell_V(const FArgs<T> &a, size_t cc, int i, int k, T *v)
 {
-    cell_V(a, cc, i, k, v, a.NS);
+    T u[9];
+    const unsigned off = (unsigned)(cc * sizeof(T));
+#pragma unroll
+    for (int m = 0; m < 9; m++) u[m] = *comp_at(a.Uin, m * a.NS, off);
+    cell_V_u(a, u, i, k, v);
 }
 /* cell (i, j, k) of any k; ZGL: the batch layout's z-side ghosts */
 template <bool ZGL, typename T>
@@ -1032,12 +1053,16 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
         if (pre && is_cell) {
             /* ZGL: past the end of a partial tile the own column is the
              * packed z-side ghost */
-            const bool zg = ZGL && ckm == g.NZ - 1;
-            const size_t cp = zg ? ZG(1, i + 2, cjm) : SI(i + 2, cjm, ckm);
-            const size_t st = zg ? (size_t)1 : NS;
             ub_load(a, i + 2, ckm, rub);
+            if (ZGL && ckm == g.NZ - 1) {
+                const size_t cp = ZG(1, i + 2, cjm);
 #pragma unroll
-            for (int m = 0; m < 9; m++) raw[m] = a.Uin[m * st + cp];
+                for (int m = 0; m < 9; m++) raw[m] = a.Uin[cp + m];
+            } else {
+                const unsigned off = (unsigned)(SI(i + 2, cjm, ckm) * sizeof(T));
+#pragma unroll
+                for (int m = 0; m < 9; m++) raw[m] = *comp_at(a.Uin, m * NS, off);
+            }
         }
         if (t < NHALO) {
             T v[9];
@@ -1207,9 +1232,9 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
 #pragma unroll
                 for (int m = 0; m < 9; m++) acc[m] = acc[m] + (g6 * sz1[m * NZS + qp] - g5 * sz1[m * NZS + qm]);
             }
-            const size_t cc = SI(i, gj, gk);
+            const unsigned off = (unsigned)(SI(i, gj, gk) * sizeof(T));
 #pragma unroll
-            for (int m = 0; m < 9; m++) a.div[m * NS + cc] = acc[m];
+            for (int m = 0; m < 9; m++) *comp_at(a.div, m * NS, off) = acc[m];
         } else if (is_cell) {
 #pragma unroll
             for (int m = 0; m < 9; m++) fx_prev[m] = fxn[m];
