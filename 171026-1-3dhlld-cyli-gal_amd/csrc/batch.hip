@@ -283,7 +283,8 @@ struct cylmhd_batch {
      * the stage launch's grid; the work list holds the items of one shape
      * contiguously, groups[s] = [first, end) of shape s */
     std::vector<int> dshape;
-    int seglen = 0, grid = 1024;
+    int seglen = 0, grid = 0;  /* grid 0: the blocks resident at once */
+    int cus = 256;             /* compute units of the device */
     int groups[kShapes][2] = {};
     /* device side */
     FArgs<float> *d_descs = nullptr;
@@ -481,13 +482,29 @@ int build(cylmhd_batch *b, const std::vector<BDisc> *keep)
     return CYLMHD_OK;
 }
 
+/* blocks of a kernel resident on the whole device at once, a multiple of 8
+ * (the XCD remap needs one); 0 if the runtime cannot say */
+int resident_blocks(cylmhd_batch *b, const void *fn, int threads)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess) return 0;
+    return per_cu * b->cus / 8 * 8;
+}
+
 template <int LIM, bool HLL, int S, bool ALT>
 void launch_shape(cylmhd_batch *b, const FArgs<float> *descs)
 {
     if constexpr (shape_built(LIM, HLL, S)) {
         const int lo = b->groups[S][0], hi = b->groups[S][1];
         if (hi <= lo) return;
-        const int G = std::max(8, std::min((hi - lo + 7) / 8 * 8, b->grid));
+        const void *fn = b->split ? reinterpret_cast<const void *>(&k_march_batch<LIM, HLL, S>)
+                                  : reinterpret_cast<const void *>(&k_fused_batch<LIM, HLL, S, ALT>);
+        /* grid 0: as many blocks as are resident at once, so that every
+         * block's run of items starts at launch */
+        int want = b->grid;
+        if (want <= 0) want = resident_blocks(b, fn, tile_threads<float, S>());
+        if (want <= 0) want = 1024;
+        const int G = std::max(8, std::min((hi - lo + 7) / 8 * 8, want));
         if (b->split)
             hipLaunchKernelGGL((k_march_batch<LIM, HLL, S>), dim3(G), dim3(tile_threads<float, S>()), 0, b->stream,
                                descs, b->d_items, lo, hi, b->d_bd);
@@ -619,6 +636,14 @@ int cylmhd_batch_create(const cylmhd_params *p, int n, cylmhd_batch **out)
         tot += v.fs->nstate;
     }
     if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) rc = CYLMHD_EHIP;
+    if (!rc) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            rc = CYLMHD_EHIP;
+        else
+            b->cus = cus;
+    }
     if (!rc && (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)) rc = CYLMHD_EHIP;
     if (!rc) {
         void *q;
@@ -695,7 +720,7 @@ int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
 {
     if (!b || steps < 1) return CYLMHD_EINVAL;
     static const int seglens[4] = {8, 16, 32, 0};
-    static const int grids[2] = {512, 1024};
+    static const int grids[3] = {0, 512, 1024};     /* 0: the resident blocks */
     int rc;
     /* every candidate starts from the discs' current states and step state
      * (time, dt, steps, alive: the spiral ramp and arm phase follow t, and
@@ -722,12 +747,12 @@ int cylmhd_batch_tune(cylmhd_batch *b, int steps, int *choice)
         if (sh < 2 || ((b->lim == 0 || b->lim == 1) && !b->hll)) avail.push_back(sh);
     /* 1: the stage form x one shape for every disc x segment length x grid */
     double best = 1e300;
-    int bs = 0, bl = 0, bg = 1024;
+    int bs = 0, bl = 0, bg = 0;
     bool bf = true;
     for (int form = 1; form >= 0; form--)
         for (int sh : avail)
             for (int li = 0; li < 4; li++)
-                for (int gi = 0; gi < 2; gi++) {
+                for (int gi = 0; gi < 3; gi++) {
                     b->split = form == 1;
                     b->dshape.assign(b->n, sh);
                     b->seglen = seglens[li];

@@ -167,7 +167,7 @@ def run_batch(cm, steps, n=48, seed=20171026):
     res = dict(discs=n, interior_cells=cells, steps=steps, ms=ms, value=updates / (ms * 1e-3),
                unit="cell-updates/s", alive=alive, steps_done=done,
                tuned={"common_tile": ["14x16/256", "6x32/256", "6x16/128", "10x16/192"][choice[0]],
-                      "segment": choice[1] or "whole r", "grid": choice[2], "kernels_per_stage": choice[3],
+                      "segment": choice[1] or "whole r", "grid": choice[2] or "resident blocks", "kernels_per_stage": choice[3],
                       "tile_by_class_nphi_x_nz": tiles, "tune_s": tune_s})
     log(f"[bench] batch: {res['value']:.4g} cell-updates/s over {steps} steps ({ms:.1f} ms), tuned {res['tuned']}")
     # the batch's scheme (minmod + HLLD, the batch discs' parameters) on one

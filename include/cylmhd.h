@@ -234,7 +234,8 @@ int cylmhd_batch_disc(cylmhd_batch *b, int d, cylmhd_ctx **ctx);
  * disc).  Tiles (phi x z cells, threads, i.e. waves per tile): 0: 14x16,
  * 256; 1: 6x32, 256; 2: 6x16, 128; 3: 10x16, 192 (2 and 3 for minmod and
  * MC with HLLD only).  choice (optional, 4 ints) = the common tile of the
- * first pass, segment length (0 = whole r), grid, kernels per stage (1, 2). */
+ * first pass, segment length (0 = whole r), grid (0 = as many blocks as are
+ * resident at once), kernels per stage (1, 2). */
 int cylmhd_batch_tune(cylmhd_batch *b, int steps, int choice[4]);
 /* The tile disc d runs with: tile[0..3] = shape, phi cells, z cells, threads. */
 int cylmhd_batch_disc_tile(cylmhd_batch *b, int d, int tile[4]);
