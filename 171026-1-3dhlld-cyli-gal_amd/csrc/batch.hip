@@ -93,9 +93,10 @@ __global__ __launch_bounds__((tile_threads<float, S>()), 3) void k_march_batch(
     }
 }
 
-/* the per-cell update of every disc, its cells dealt flat (k fastest, 256
- * to a block, whatever the disc's NZ): disc d's blocks are off[d] ..
- * off[d+1]-1, block b of the batch writes Setdt partial b */
+/* the per-cell update of every disc, each plane's cells dealt flat (k
+ * fastest, 64 to a wavefront, whatever the disc's NZ; four wavefronts to a
+ * block): disc d's blocks are off[d] .. off[d+1]-1, block b of the batch
+ * writes Setdt partial b */
 template <bool ALT>
 __global__ __launch_bounds__(256) void k_update_batch(const FArgs<float> *__restrict__ descs, const int *__restrict__ off,
                                                       int n, const BDisc *__restrict__ bd)
@@ -106,11 +107,12 @@ __global__ __launch_bounds__(256) void k_update_batch(const FArgs<float> *__rest
     update_cell<float, ALT, true>(a, 0, blockIdx.x - off[d], blockIdx.x);
 }
 
-/* update blocks of one disc: its interior cells, 256 to a block */
+/* update blocks of one disc: each plane's interior cells in wavefronts of
+ * 64, four wavefronts to a block */
 inline int update_blocks_of(int NX, int NY, int NZ)
 {
-    const long long cells = (long long)(NX - 2) * (NY - 2) * (NZ - 2);
-    return (int)((cells + 255) / 256);
+    const long long waves = (long long)(NX - 2) * (((long long)(NY - 2) * (NZ - 2) + 63) / 64);
+    return (int)((waves + 3) / 4);
 }
 
 /* tile shapes compiled for a limiter x solver: all four for minmod and MC

@@ -1253,9 +1253,9 @@ __device__ __forceinline__ void march_tile(const FArgs<T> &a, int tile, int seg)
  * row's nch = ceil((NZ-2)/64)), four wavefronts to a block; the stage
  * input is read once (UtoV gives the primitives the sources use, as the
  * plane loads of the march do), the divergence from a.div.  blk: the
- * block's index among this disc's update blocks of the stage.  FLAT: the
- * cells are dealt 256 to a block in (i, j, k) order instead (discs whose
- * rows are short against a wavefront).  Floor counters and the Setdt
+ * block's index among this disc's update blocks of the stage.  FLAT: each
+ * plane's cells are dealt 64 to a wavefront in (j, k) order instead (discs
+ * whose rows are short against a wavefront).  Floor counters and the Setdt
  * partial (dtpart slot `slot`) as in fused_tile. */
 template <typename T, bool ALT, bool FLAT = false, int RJ = 1>
 __device__ __forceinline__ void update_cell(const FArgs<T> &a, int nch, int blk, int slot)
@@ -1263,19 +1263,22 @@ __device__ __forceinline__ void update_cell(const FArgs<T> &a, int nch, int blk,
     const DevGeom &g = a.g;
     const size_t NS = a.NS;
     const int lane = threadIdx.x & 63;
-    const int nj = g.NY - 2, nrows = (g.NX - 2) * nj;
+    const int nj = g.NY - 2;
     int i, j, k;
     bool on;
     if constexpr (FLAT) {
-        /* the interior cells in (i, j, k) order, k fastest, 64 to a wave
-         * whatever NZ (short rows fill whole waves; i may vary in a wave) */
-        const unsigned nk = (unsigned)(g.NZ - 2);
-        const unsigned q = (unsigned)blk * blockDim.x + threadIdx.x;
+        /* each plane's interior cells in (j, k) order, k fastest, 64 to a
+         * wave whatever NZ (short rows fill whole waves), no wave crossing
+         * a plane: i is uniform in the wave, its table row a scalar load */
+        const unsigned nk = (unsigned)(g.NZ - 2), P = (unsigned)nj * nk, wpp = (P + 63) / 64;
+        const unsigned w = (unsigned)__builtin_amdgcn_readfirstlane((int)(((unsigned)blk * blockDim.x + threadIdx.x) >> 6));
+        const unsigned ip = w / wpp;
+        const unsigned q = (w - ip * wpp) * 64 + (unsigned)lane;
         const unsigned r = q / nk;
-        on = r < (unsigned)nrows;
+        on = ip < (unsigned)(g.NX - 2) && q < P;
+        i = 1 + (int)ip;
+        j = 1 + (int)r;
         k = 1 + (int)(q - r * nk);
-        i = 1 + (int)(r / (unsigned)nj);
-        j = 1 + (int)(r % (unsigned)nj);
     } else {
         /* RJ consecutive rows j of one plane per wavefront (nphi is even:
          * the axis map j <-> j + nphi/2 needs it) */
@@ -1386,6 +1389,19 @@ __device__ __forceinline__ size_t halo_cells(const DevGeom &g)
     return 2 * NY * NZ + (NX - 2) * 2 * NZ + (NX - 2) * (NY - 2) * 2;
 }
 
+/* ghost cell r of the list: the r-side planes i = 0, NX-1 (all j, k), then
+ * the phi sides j = 0, NY-1 (interior i, all k), then the z sides k = 0,
+ * NZ-1 (interior i, j); I: the integer type of the division (32-bit where
+ * the count fits: a 64-bit division is a long software sequence) */
+template <typename I>
+__device__ __forceinline__ void halo_ijk(I r, I NX, I NY, I NZ, int &i, int &j, int &k)
+{
+    const I nA = 2 * NY * NZ, nB = (NX - 2) * 2 * NZ;
+    if (r < nA) { i = (r / (NY * NZ)) ? (int)NX - 1 : 0; r %= NY * NZ; j = (int)(r / NZ); k = (int)(r % NZ); }
+    else if ((r -= nA) < nB) { i = 1 + (int)(r / (2 * NZ)); r %= 2 * NZ; j = (r / NZ) ? (int)NY - 1 : 0; k = (int)(r % NZ); }
+    else { r -= nB; i = 1 + (int)(r / ((NY - 2) * 2)); r %= (NY - 2) * 2; j = 1 + (int)(r / 2); k = (r % 2) ? (int)NZ - 1 : 0; }
+}
+
 /* ghost cells first, first + stride, ... of the composed boundary map;
  * every lane of the wave must call it (the counters are wave-reduced) */
 template <typename T, bool ZGL = false>
@@ -1395,13 +1411,12 @@ __device__ __forceinline__ void halo_range(const FArgs<T> &a, size_t first, size
     const size_t NX = g.NX, NY = g.NY, NZ = g.NZ, N = a.N, NS = a.NS;
     const size_t nA = 2 * NY * NZ, nB = (NX - 2) * 2 * NZ, nC = (NX - 2) * (NY - 2) * 2;
     const size_t tot = nA + nB + nC;
+    const bool narrow = tot + stride <= 0xffffffffull;
     int f0 = 0, f1 = 0, f2 = 0;
     for (size_t t = first; t < tot; t += stride) {
         int i, j, k;
-        size_t r = t;
-        if (r < nA) { i = (r / (NY * NZ)) ? (int)NX - 1 : 0; r %= NY * NZ; j = (int)(r / NZ); k = (int)(r % NZ); }
-        else if ((r -= nA) < nB) { i = 1 + (int)(r / (2 * NZ)); r %= 2 * NZ; j = (r / NZ) ? (int)NY - 1 : 0; k = (int)(r % NZ); }
-        else { r -= nB; i = 1 + (int)(r / ((NY - 2) * 2)); r %= (NY - 2) * 2; j = 1 + (int)(r / 2); k = (r % 2) ? (int)NZ - 1 : 0; }
+        if (narrow) halo_ijk<unsigned>((unsigned)t, (unsigned)NX, (unsigned)NY, (unsigned)NZ, i, j, k);
+        else halo_ijk<size_t>(t, NX, NY, NZ, i, j, k);
         int si, sj, sk;
         unsigned sign;
         bool zero3;
