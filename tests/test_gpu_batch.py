@@ -182,6 +182,19 @@ def test_batch_other_limiters_match_single_discs(gpu, limiter):
     compare_with_single(ps)
 
 
+@pytest.mark.parametrize("solver,rk", [(cm.SOLVER_HLL, 3), (cm.SOLVER_HLLD, 2), (cm.SOLVER_HLL, 2)],
+                         ids=["hll-rk3", "hlld-rk2", "hll-rk2"])
+def test_batch_hll_and_rk2_match_single_discs(gpu, solver, rk):
+    """The batch with the HLL solver (its kernels are built for two tile
+    shapes) and with RK2 (two stages, the RK2 z boundary and z absorbing
+    layer in the ghost pass) runs what each disc runs alone."""
+    ps = batch_params()
+    for p in ps:
+        p.solver = solver
+        p.rk = rk
+    compare_with_single(ps)
+
+
 def test_batch_rejects_mixed_limiters(gpu):
     ps = batch_params()
     ps[1].limiter = cm.LIMITER_MC
