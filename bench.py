@@ -24,9 +24,12 @@ roofline  = the RK stage, two kernels (the r-march: limiter states, HLLD
 precision = relative L1 / Linf error per conserved variable of the FP32
             path against the plain FP64 path after 100 steps on the same
             dt sequence, with div B and floor hits (precision.py).
-batch     = the batching form: 48 uneven discs advanced together, one launch
-            per RK stage over a tile work list, per-disc dt on the device,
-            the step replayed from a hipGraph, launch shape autotuned.
+batch     = the batching form: 48 uneven discs advanced together, per RK
+            stage one r-march launch over a tile work list of every disc and
+            one update launch over every disc's cells, per-disc dt on the
+            device, the step replayed from a hipGraph, launch shape
+            autotuned; compared with the batch's scheme on one 512x512x256
+            disc (single_disc_same_scheme).
 cpu_baseline = the CPU restatement of the reference (the oracle: the
             reference's data layout and pass structure, OpenMP on all host
             cores) on the 256x256x128 disc of the FP64 parity runs (MC +
